@@ -1,0 +1,341 @@
+"""ctypes bindings of libargweaver_amd.so.
+
+Mirrors the reference's ``argweaver/argweaverc.py`` (lines 29-343 declare
+the exports, 529-744 wrap them) for the threading path: the same C entry
+points under the same names, called with the same array conventions
+(``int**`` row-pointer matrices, ``char**`` sequences, opaque
+``LocalTrees*`` handles).  The reference wraps arglib ARG objects; here the
+ARG travels as plain arrays (``ptrees``, ``ages``, ``sprs``, ``blocklens``),
+which is exactly what ``arg2ctrees``/``ctrees2arg`` hand to and read from
+the C library (argweaverc.py:1285-1353).
+"""
+import ctypes
+from ctypes import (POINTER, c_bool, c_char_p, c_double, c_int, c_longlong,
+                    c_ulonglong, c_void_p)
+
+import numpy as np
+
+from . import load_library
+
+c_int_p = POINTER(c_int)
+c_int_p_p = POINTER(POINTER(c_int))
+c_double_p = POINTER(c_double)
+c_double_p_p = POINTER(POINTER(c_double))
+c_char_p_p = POINTER(c_char_p)
+intstate = c_int * 2
+
+_lib = load_library()
+
+
+def _export(name, restype, argtypes):
+    fn = getattr(_lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+# ARG data structure API (argweaverc.py:300-331)
+argweaver_new_trees = _export(
+    "arghmm_new_trees", c_void_p,
+    [c_int_p_p, c_int_p_p, c_int_p_p, c_int_p, c_int, c_int, c_int])
+argweaver_copy_trees = _export("arghmm_copy_trees", c_void_p, [c_void_p])
+get_local_trees_ntrees = _export("get_local_trees_ntrees", c_int, [c_void_p])
+get_local_trees_nnodes = _export("get_local_trees_nnodes", c_int, [c_void_p])
+get_local_trees_ptrees = _export(
+    "get_local_trees_ptrees", None,
+    [c_void_p, c_int_p_p, c_int_p_p, c_int_p_p, c_int_p])
+get_local_trees_blocks = _export(
+    "get_local_trees_blocks", None, [c_void_p, c_int_p, c_int_p])
+get_treelens = _export(
+    "get_treelens", None, [c_void_p, c_double_p, c_int, c_double_p])
+delete_local_trees = _export("delete_local_trees", None, [c_void_p])
+
+# thread data structures (argweaverc.py:333-343)
+argweaver_get_nstates = _export(
+    "arghmm_get_nstates", None, [c_void_p, c_int, c_bool, c_int_p])
+get_state_spaces = _export(
+    "get_state_spaces", POINTER(POINTER(intstate)), [c_void_p, c_int, c_bool])
+delete_state_spaces = _export(
+    "delete_state_spaces", None, [POINTER(POINTER(intstate)), c_int])
+delete_path = _export("delete_path", None, [c_void_p])
+setLogLevel = _export("setLogLevel", None, [c_int])
+
+# forward algorithm and thread sampling (argweaverc.py:165-243)
+argweaver_forward_alg = _export(
+    "arghmm_forward_alg", c_double_p_p,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, c_bool, c_double_p, c_bool, c_bool])
+delete_forward_matrix = _export(
+    "delete_forward_matrix", None, [c_double_p_p, c_int])
+delete_double_matrix = _export(
+    "delete_double_matrix", None, [c_double_p_p, c_int])
+argweaver_sample_posterior = _export(
+    "arghmm_sample_posterior", POINTER(intstate),
+    [c_int_p_p, c_int_p_p, c_int_p_p, c_int_p, c_int, c_int,
+     c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, POINTER(intstate)])
+argweaver_sample_thread = _export(
+    "arghmm_sample_thread", c_void_p,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int])
+argweaver_sample_arg_seq = _export(
+    "arghmm_sample_arg_seq", c_void_p,
+    [c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int])
+argweaver_sample_arg_refine = _export(
+    "arghmm_sample_arg_refine", c_void_p,
+    [c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, c_int, c_int])
+argweaver_resample_arg = _export(
+    "arghmm_resample_arg", c_void_p,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, c_int, c_int])
+
+# extensions of this library
+awamd_remove_thread = _export("awamd_remove_thread", c_void_p, [c_void_p, c_int])
+awamd_thread_forward = _export(
+    "awamd_thread_forward", c_int,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p,
+     c_int, c_double_p, c_int, c_double_p, c_int_p, c_double_p, c_double_p])
+awamd_num_column_states = _export(
+    "awamd_num_column_states", c_longlong, [c_void_p, c_int, c_int_p])
+awamd_get_mappings = _export("awamd_get_mappings", c_int, [c_void_p, c_int_p])
+awamd_get_seqids = _export("awamd_get_seqids", c_int, [c_void_p, c_int_p])
+awamd_sample_recombinations = _export(
+    "awamd_sample_recombinations", c_int,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_int_p,
+     c_int, c_int_p, c_int_p, c_int_p])
+awamd_add_thread = _export(
+    "awamd_add_thread", c_void_p,
+    [c_void_p, c_int, c_int_p, c_int, c_int, c_int_p, c_int_p, c_int_p])
+awamd_simulate = _export(
+    "awamd_simulate", c_void_p,
+    [c_int, c_int, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_ulonglong, ctypes.c_char_p])
+
+_libc = ctypes.CDLL(None)
+
+
+def set_random_seed(num):
+    """Seed the process-global C rand() stream (argweaverc.py:352-354)."""
+    _libc.srand(int(num))
+
+
+#=============================================================================
+# array helpers (ctypes_export.py:29-35 c_matrix conventions)
+
+def _dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(c_int_p)
+
+
+class IntMatrix(object):
+    """An int** view over a C-contiguous 2-D int32 array (kept alive here)."""
+
+    def __init__(self, arr):
+        self.arr = np.ascontiguousarray(arr, dtype=np.int32)
+        rows = [self.arr[i].ctypes.data_as(c_int_p) for i in range(self.arr.shape[0])]
+        self.rows = (c_int_p * max(len(rows), 1))(*rows)
+
+    @property
+    def ptr(self):
+        return ctypes.cast(self.rows, c_int_p_p)
+
+
+class CSeqs(object):
+    """A char** over equal-length byte rows (kept alive here)."""
+
+    def __init__(self, seqs):
+        if isinstance(seqs, np.ndarray):
+            self.arr = np.ascontiguousarray(seqs, dtype=np.uint8)
+        else:
+            rows = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+            self.arr = np.frombuffer(b"".join(rows), dtype=np.uint8).reshape(len(rows), -1).copy()
+        self.nseqs, self.seqlen = self.arr.shape
+        ptrs = [self.arr[i].ctypes.data for i in range(self.nseqs)]
+        self.rows = (c_char_p * self.nseqs)(*[ctypes.cast(p, c_char_p) for p in ptrs])
+
+    @property
+    def ptr(self):
+        return ctypes.cast(self.rows, c_char_p_p)
+
+
+def new_trees(ptrees, ages, sprs, blocklens, start_coord=0):
+    """arghmm_new_trees over numpy arrays [ntrees, nnodes] / [ntrees, 4] / [ntrees]."""
+    ptrees = np.asarray(ptrees, dtype=np.int32)
+    ntrees, nnodes = ptrees.shape
+    P, A, S = IntMatrix(ptrees), IntMatrix(ages), IntMatrix(sprs)
+    bl = np.ascontiguousarray(blocklens, dtype=np.int32)
+    return argweaver_new_trees(P.ptr, A.ptr, S.ptr, _iptr(bl), ntrees, nnodes, start_coord)
+
+
+def trees_arrays(trees):
+    """get_local_trees_ptrees as numpy arrays (ptrees, ages, sprs, blocklens)."""
+    ntrees = get_local_trees_ntrees(trees)
+    nnodes = get_local_trees_nnodes(trees)
+    P = IntMatrix(np.zeros((ntrees, nnodes), np.int32))
+    A = IntMatrix(np.zeros((ntrees, nnodes), np.int32))
+    S = IntMatrix(np.zeros((ntrees, 4), np.int32))
+    bl = np.zeros(ntrees, np.int32)
+    get_local_trees_ptrees(trees, P.ptr, A.ptr, S.ptr, _iptr(bl))
+    return P.arr, A.arr, S.arr, bl
+
+
+def trees_internal(trees):
+    """Block arrays in internal node numbering plus mappings and seqids."""
+    ntrees = get_local_trees_ntrees(trees)
+    nnodes = get_local_trees_nnodes(trees)
+    maps = np.zeros((ntrees, nnodes), np.int32)
+    awamd_get_mappings(trees, _iptr(maps))
+    nleaves = (nnodes + 1) // 2
+    seqids = np.zeros(max(nleaves, 1), np.int32)
+    awamd_get_seqids(trees, _iptr(seqids))
+    return maps, seqids[:nleaves]
+
+
+def num_column_states(trees, ntimes):
+    ntrees = get_local_trees_ntrees(trees)
+    S = np.zeros(ntrees, np.int32)
+    W = awamd_num_column_states(trees, ntimes, _iptr(S))
+    return int(W), S
+
+
+def get_nstates(trees, ntimes, seqlen):
+    out = np.zeros(seqlen, np.int32)
+    argweaver_get_nstates(trees, ntimes, False, _iptr(out))
+    return out
+
+
+def state_spaces(trees, ntimes):
+    """get_state_spaces as a list of [S, 2] arrays (node, time)."""
+    ntrees = get_local_trees_ntrees(trees)
+    ptr = get_state_spaces(trees, ntimes, False)
+    _, S = num_column_states(trees, ntimes)
+    out = []
+    for i in range(ntrees):
+        rows = np.ctypeslib.as_array(ctypes.cast(ptr[i], c_int_p), shape=(int(S[i]) * 2,))
+        out.append(rows.reshape(-1, 2).copy())
+    delete_state_spaces(ptr, ntrees)
+    return out
+
+
+#=============================================================================
+# high-level wrappers (argweaverc.py:529-744 counterparts)
+
+def forward_algorithm(trees, seqs, times, popsizes, rho, mu, prior=None, slow=False):
+    """argweaver_forward_algorithm: the forward table as a list of rows."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    pr = None if prior is None else np.ascontiguousarray(prior, np.float64)
+    fw = argweaver_forward_alg(
+        trees, _dptr(times), len(times), _dptr(popsizes), rho, mu, cs.ptr,
+        cs.nseqs, cs.seqlen, pr is not None,
+        _dptr(pr) if pr is not None else None, False, slow)
+    if not fw:
+        raise RuntimeError("arghmm_forward_alg failed")
+    nst = get_nstates(trees, len(times), cs.seqlen)
+    rows = [np.ctypeslib.as_array(fw[i], shape=(max(int(nst[i]), 1),)).copy()
+            for i in range(cs.seqlen)]
+    delete_forward_matrix(fw, cs.seqlen)
+    return rows
+
+
+def sample_thread(trees, seqs, times, popsizes, rho, mu):
+    """arghmm_sample_thread: thread the last sequence into the ARG (in place)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_sample_thread(trees, _dptr(times), len(times), _dptr(popsizes),
+                                   rho, mu, cs.ptr, cs.nseqs, cs.seqlen)
+
+
+def resample_arg(trees, seqs, times, popsizes, rho, mu, niters=1, nremove=1):
+    """arghmm_resample_arg: Gibbs re-threading sweeps (in place)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_resample_arg(trees, _dptr(times), len(times), _dptr(popsizes),
+                                  rho, mu, cs.ptr, cs.nseqs, cs.seqlen, niters, nremove)
+
+
+def sample_arg_seq(seqs, times, popsizes, rho, mu):
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_sample_arg_seq(_dptr(times), len(times), _dptr(popsizes), rho, mu,
+                                    cs.ptr, cs.nseqs, cs.seqlen)
+
+
+def thread_forward(trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=None,
+                   precision=0, want_alpha=True):
+    """Forward (+ traceback with explicit uniforms) on the device.
+
+    Returns dict(alpha=[W] FP64 block-major table or None, path=[L] or None,
+    lnl=float, ms=[prep, emit, forward, traceback])."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    W, _ = num_column_states(trees, len(times))
+    L = int(sum(trees_arrays(trees)[3]))
+    alpha = np.zeros(W, np.float64) if want_alpha else None
+    path = np.zeros(L, np.int32) if uniforms is not None else None
+    u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+    lnl = c_double(0.0)
+    ms = np.zeros(4, np.float64)
+    rc = awamd_thread_forward(
+        trees, _dptr(times), len(times), _dptr(popsizes), rho, mu, cs.ptr, new_chrom,
+        _dptr(u) if u is not None else None, precision,
+        _dptr(alpha) if alpha is not None else None,
+        _iptr(path) if path is not None else None, ctypes.byref(lnl), _dptr(ms))
+    if rc != 0:
+        raise RuntimeError("awamd_thread_forward failed (%d)" % rc)
+    return dict(alpha=alpha, path=path, lnl=lnl.value, ms=ms)
+
+
+def simulate(k, seqlen, times, popsizes, rho, mu, seed):
+    """Seeded DSMC workload: returns (trees handle with k leaves, seqs [k, L] uint8)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    seqs = np.zeros((k, seqlen), np.uint8)
+    buf = seqs.ctypes.data_as(ctypes.c_char_p)
+    trees = awamd_simulate(k, seqlen, _dptr(times), len(times), _dptr(popsizes), rho, mu,
+                           seed, buf)
+    if not trees:
+        raise RuntimeError("simulation failed")
+    return trees, seqs
+
+
+def remove_thread(trees, seqid):
+    return awamd_remove_thread(trees, seqid)
+
+
+def sample_recombinations(trees, times, popsizes, rho, mu, path, maxrecomb=1 << 20):
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    path = np.ascontiguousarray(path, np.int32)
+    cap = min(maxrecomb, len(path))
+    pos = np.zeros(cap, np.int32)
+    node = np.zeros(cap, np.int32)
+    tm = np.zeros(cap, np.int32)
+    n = awamd_sample_recombinations(trees, _dptr(times), len(times), _dptr(popsizes), rho,
+                                    mu, _iptr(path), cap, _iptr(pos), _iptr(node), _iptr(tm))
+    return pos[:n], node[:n], tm[:n]
+
+
+def add_thread(trees, ntimes, path, seqid, recomb_pos, recomb_node, recomb_time):
+    path = np.ascontiguousarray(path, np.int32)
+    rp = np.ascontiguousarray(recomb_pos, np.int32)
+    rn = np.ascontiguousarray(recomb_node, np.int32)
+    rt = np.ascontiguousarray(recomb_time, np.int32)
+    return awamd_add_thread(trees, ntimes, _iptr(path), seqid, len(rp), _iptr(rp), _iptr(rn),
+                            _iptr(rt))
+
+
+def get_time_points(ntimes=20, maxtime=200e3, delta=.01):
+    """Time grid (argweaver/__init__.py:97-105, model.h:21-32)."""
+    return [(np.exp(i / float(ntimes - 1) * np.log(1 + delta * maxtime)) - 1) / delta
+            for i in range(ntimes)]

@@ -1,0 +1,605 @@
+// capi.cpp -- the C ABI: the reference's arghmm_* entry points for the
+// threading path, backed by the gfx950 engine, plus the handle/support
+// symbols callers use around them.  Host work here is what the reference
+// also does on the host between HMM passes: flattening the ARG, drawing
+// the traceback's uniforms from the process-global glibc rand() stream,
+// sampling recombination points and splicing the thread into the ARG.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/argweaver_amd.h"
+#include "engine.h"
+#include "local_trees.h"
+
+namespace aw {
+
+static int g_log_level = 0;
+
+// frand / expovariate (common.h:103-125) on the process-global glibc stream
+static inline double frand() { return rand() / double(RAND_MAX); }
+
+// sample() (common.h:272-290)
+static int sample_weights(const double *w, int n)
+{
+    double total = 0.0;
+    for (int i = 0; i < n; i++) total += w[i];
+    double pick = frand() * total;
+    double x = 0.0;
+    for (int i = 0; i < n; i++) {
+        x += w[i];
+        if (x >= pick) return i;
+    }
+    return n - 1;
+}
+
+static void build_instance(const ArgTrees &tr, const double *times, int ntimes,
+                           const double *popsizes, double rho, double mu, char **seqs,
+                           int new_chrom, HostInstance &hi)
+{
+    flatten(tr, hi.ft);
+    const int nleaves = tr.num_leaves();
+    hi.rows.resize(nleaves + 1);
+    for (int i = 0; i < nleaves; i++) hi.rows[i] = seqs[tr.seqids[i]] + tr.start_coord;
+    hi.rows[nleaves] = seqs[new_chrom] + tr.start_coord;
+    std::vector<double> pop(ntimes);
+    for (int i = 0; i < ntimes; i++) pop[i] = popsizes ? popsizes[i] : 0.0;
+    model_init(hi.model, ntimes, times, pop.data(), rho, mu);
+}
+
+// sample_recombinations (recomb.cpp:151-235) for external threading
+static void sample_recombinations(const ArgTrees &tr, const Model &m, const int *path,
+                                  std::vector<int> &rpos, std::vector<int> &rnode,
+                                  std::vector<int> &rtime)
+{
+    const int T = m.T;
+    std::vector<int> snode, stime, par, c0, c1, age;
+    std::vector<int> cn, ct;
+    std::vector<double> probs;
+    int end = tr.start_coord;
+    for (size_t bi = 0; bi < tr.blocks.size(); bi++) {
+        const LBlock &B = tr.blocks[bi];
+        const LTree &lt = B.tree;
+        const int N = lt.nnodes;
+        par.resize(N); c0.resize(N); c1.resize(N); age.resize(N);
+        for (int i = 0; i < N; i++) {
+            par[i] = lt.nodes[i].parent;
+            c0[i] = lt.nodes[i].child[0];
+            c1[i] = lt.nodes[i].child[1];
+            age[i] = lt.nodes[i].age;
+        }
+        TreeView t{N, lt.root, par.data(), c0.data(), c1.data(), age.data()};
+        Lineages L;
+        count_lineages(t, T, L);
+        TransVec tv;
+        calc_trans(t, m, L, tv);
+        tree_states(lt, T, snode, stime);
+
+        int start = end;
+        end += B.blocklen;
+        int next_recomb = -1;
+        // the first column of every block is a switch (or the very first column)
+        for (int i = start + 1; i < end; i++) {
+            const int pi = path[i - tr.start_coord], pl = path[i - 1 - tr.start_coord];
+            if (pi == pl) {
+                if (i > next_recomb) {
+                    int a = stime[pl];
+                    double self = tv.get_time(a, a, age[snode[pl]], true);
+                    double rate = 1.0 - (tv.norecombs[a] / self);
+                    double ev = -log(frand()) / rate;
+                    next_recomb = int(fmin(double(end), i + ev));
+                }
+                if (i < next_recomb) continue;
+            }
+            next_recomb = -1;
+            const int sn = snode[pi], st = stime[pi], ln = snode[pl], ltm = stime[pl];
+            // get_possible_recomb (recomb.cpp:124-148)
+            cn.clear();
+            ct.clear();
+            int end_time = st < ltm ? st : ltm;
+            if (sn == ln)
+                for (int k = age[sn]; k <= end_time; k++) { cn.push_back(sn); ct.push_back(k); }
+            for (int k = 0; k <= end_time; k++) { cn.push_back(-1); ct.push_back(k); }
+            // recomb_prob_unnormalized (recomb.cpp:14-119)
+            probs.clear();
+            for (size_t x = 0; x < cn.size(); x++) {
+                int k = ct[x], j = st;
+                int root_time = age[lt.root] > ltm ? age[lt.root] : ltm;
+                int rpa = (cn[x] == -1 || par[cn[x]] == -1 || cn[x] == ln) ? ltm : age[par[cn[x]]];
+                int nbk = L.nb[k] + int(k < ltm);
+                int nrk = L.nr[k] + int(k <= ltm) + int(k == ltm) - int(k == root_time);
+                double precomb = nbk * m.steps[k] / nrk;
+                double sum = 0.0;
+                for (int y = k; y < j - 1; y++) {
+                    int nbm = L.nb[y] + int(y < ltm) - int(y < rpa);
+                    sum += (m.steps[y] * nbm / (2.0 * m.pop[y]));
+                }
+                double pcoal = 1.0;
+                int nbj = L.nb[j] + int(j < ltm) - int(j < rpa);
+                if (k == j) {
+                    if (j < T - 2) pcoal = 1.0 - exp(-m.csteps[2 * j] * nbj / (2.0 * m.pop[j]));
+                } else {
+                    int y = j - 1;
+                    int nbm = L.nb[y] + int(y < ltm) - int(y < rpa);
+                    sum += (m.csteps[2 * y] * nbm / (2.0 * m.pop[y]));
+                    if (j < T - 2)
+                        pcoal = 1.0 - exp(-m.csteps[2 * j - 1] * nbm / (2.0 * m.pop[y]) -
+                                          m.csteps[2 * j] * nbj / (2.0 * m.pop[j]));
+                }
+                int ncj = L.nc[j] - int(j <= rpa) - int(j == rpa) + int(j <= ltm) + int(j == ltm);
+                pcoal /= ncj;
+                probs.push_back(precomb * exp(-sum) * pcoal);
+            }
+            int pick = sample_weights(probs.data(), (int)probs.size());
+            rpos.push_back(i);
+            rnode.push_back(cn[pick]);
+            rtime.push_back(ct[pick]);
+        }
+    }
+}
+
+// sample_arg_thread (sample_thread.cpp:578-633)
+static int sample_arg_thread(ArgTrees &tr, const double *times, int ntimes, const double *popsizes,
+                             double rho, double mu, char **seqs, int new_chrom)
+{
+    HostInstance hi;
+    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    const int L = tr.length();
+    std::vector<double> u(L);
+    for (int i = 0; i < L; i++) u[i] = frand();   // one draw per column, last column first
+    std::vector<int> path(L);
+    RunStats st;
+    if (run_instance(hi, u.data(), path.data(), nullptr, nullptr, &st)) return 1;
+    if (g_log_level >= 1)
+        fprintf(stderr, "forward+trace on %s: prep %.3f ms, emit %.3f ms, fw %.3f ms, tb %.3f ms (%d blocks)\n",
+                device_name(), st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback,
+                hi.ft.nblocks);
+    std::vector<int> rpos, rnode, rtime;
+    sample_recombinations(tr, hi.model, path.data(), rpos, rnode, rtime);
+    add_arg_thread(tr, ntimes, path.data(), new_chrom, rpos, rnode, rtime);
+    return 0;
+}
+
+static void require_gpu(const char *fn)
+{
+    if (!device_available()) {
+        fprintf(stderr, "argweaver_amd: %s needs a HIP device and none is visible\n", fn);
+        abort();
+    }
+}
+
+}  // namespace aw
+
+using namespace aw;
+
+extern "C" {
+
+//---------------------------------------------------------------------------
+// handles
+
+LocalTrees *arghmm_new_trees(int **ptrees, int **ages, int **sprs, int *blocklens, int ntrees,
+                             int nnodes, int start_coord)
+{
+    return reinterpret_cast<LocalTrees *>(
+        trees_from_arrays(ptrees, ages, sprs, blocklens, ntrees, nnodes, start_coord));
+}
+
+LocalTrees *arghmm_copy_trees(LocalTrees *trees)
+{
+    return reinterpret_cast<LocalTrees *>(new ArgTrees(*reinterpret_cast<ArgTrees *>(trees)));
+}
+
+int get_local_trees_ntrees(LocalTrees *trees)
+{
+    return (int)reinterpret_cast<ArgTrees *>(trees)->blocks.size();
+}
+
+int get_local_trees_nnodes(LocalTrees *trees)
+{
+    return reinterpret_cast<ArgTrees *>(trees)->nnodes;
+}
+
+void get_local_trees_ptrees(LocalTrees *trees, int **ptrees, int **ages, int **sprs, int *blocklens)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    const int nleaves = tr.num_leaves();
+    std::vector<int> perm(tr.nnodes);
+    for (int i = 0; i < nleaves; i++) perm[i] = tr.seqids[i];
+    for (int i = nleaves; i < tr.nnodes; i++) perm[i] = i;
+    if (!check_trees(tr)) {
+        fprintf(stderr, "argweaver_amd: get_local_trees_ptrees on an invalid ARG\n");
+        abort();
+    }
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        const LBlock &B = tr.blocks[i];
+        for (int j = 0; j < B.tree.nnodes; j++) {
+            int parent = B.tree.nodes[j].parent;
+            if (parent != -1) parent = perm[parent];
+            ptrees[i][perm[j]] = parent;
+            ages[i][perm[j]] = B.tree.nodes[j].age;
+        }
+        blocklens[i] = B.blocklen;
+        if (!spr_is_null(B.spr)) {
+            sprs[i][0] = perm[B.spr.rn];
+            sprs[i][1] = B.spr.rt;
+            sprs[i][2] = perm[B.spr.cn];
+            sprs[i][3] = B.spr.ct;
+        } else {
+            sprs[i][0] = B.spr.rn;
+            sprs[i][1] = B.spr.rt;
+            sprs[i][2] = B.spr.cn;
+            sprs[i][3] = B.spr.ct;
+        }
+    }
+}
+
+void get_local_trees_blocks(LocalTrees *trees, int *starts, int *ends)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    int end = tr.start_coord;
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        starts[i] = end;
+        end += tr.blocks[i].blocklen;
+        ends[i] = end;
+    }
+}
+
+void get_treelens(LocalTrees *trees, double *times, int ntimes, double *treelens)
+{
+    (void)ntimes;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        const LTree &t = tr.blocks[i].tree;
+        double len = 0.0;
+        for (int j = 0; j < t.nnodes; j++) {
+            int p = t.nodes[j].parent;
+            if (p != -1) len += times[t.nodes[p].age] - times[t.nodes[j].age];
+        }
+        treelens[i] = len;
+    }
+}
+
+void delete_local_trees(LocalTrees *trees) { delete reinterpret_cast<ArgTrees *>(trees); }
+
+//---------------------------------------------------------------------------
+// state spaces
+
+void arghmm_get_nstates(LocalTrees *trees, int ntimes, bool internal, int *nstates)
+{
+    if (internal) {
+        fprintf(stderr, "argweaver_amd: internal-branch state spaces are not supported\n");
+        abort();
+    }
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    std::vector<int> sn, st;
+    int i = 0;
+    for (const LBlock &B : tr.blocks) {
+        tree_states(B.tree, ntimes, sn, st);
+        for (int j = 0; j < B.blocklen; j++) nstates[i + j] = (int)sn.size();
+        i += B.blocklen;
+    }
+}
+
+intstate **get_state_spaces(LocalTrees *trees, int ntimes, bool internal)
+{
+    (void)internal;   // the reference ignores it here (states.cpp:230-251)
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    intstate **all = new intstate *[tr.blocks.size()];
+    std::vector<int> sn, st;
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        tree_states(tr.blocks[i].tree, ntimes, sn, st);
+        all[i] = new intstate[sn.size()];
+        for (size_t j = 0; j < sn.size(); j++) {
+            all[i][j][0] = sn[j];
+            all[i][j][1] = st[j];
+        }
+    }
+    return all;
+}
+
+void delete_state_spaces(intstate **all_states, int ntrees)
+{
+    for (int i = 0; i < ntrees; i++) delete[] all_states[i];
+    delete[] all_states;
+}
+
+//---------------------------------------------------------------------------
+// threading HMM
+
+double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                            double rho, double mu, char **seqs, int nseqs, int seqlen,
+                            bool prior_given, double *prior, bool internal, bool slow)
+{
+    (void)slow;   // the dense variant computes the same table; one device kernel serves both
+    if (internal) {
+        fprintf(stderr, "argweaver_amd: internal-branch forward is not supported\n");
+        return nullptr;
+    }
+    require_gpu("arghmm_forward_alg");
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    HostInstance hi;
+    const int new_chrom = tr.num_leaves();   // ArgHmmMatrixIter default (matrices.h:224-226)
+    if (new_chrom >= nseqs) {
+        fprintf(stderr, "argweaver_amd: arghmm_forward_alg needs %d sequences\n", new_chrom + 1);
+        return nullptr;
+    }
+    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    std::vector<int> Sv;
+    long long W = instance_states(hi.ft, ntimes, &Sv);
+    double *table = new double[W > 0 ? W : 1];
+    if (run_instance(hi, nullptr, nullptr, table, nullptr, nullptr, Precision::FP64,
+                     prior_given ? prior : nullptr)) {
+        delete[] table;
+        return nullptr;
+    }
+    // rows of the caller's table alias the contiguous block-major buffer
+    double **fw = new double *[seqlen > 0 ? seqlen : 1];
+    for (int i = 0; i < seqlen; i++) fw[i] = nullptr;
+    long long off = 0;
+    int pos = tr.start_coord;
+    for (int b = 0; b < hi.ft.nblocks; b++) {
+        for (int i = 0; i < hi.ft.len[b]; i++, pos++)
+            if (pos >= 0 && pos < seqlen) fw[pos] = table + off + (long long)i * Sv[b];
+        off += (long long)hi.ft.len[b] * Sv[b];
+    }
+    if (seqlen > 0 && fw[0] != table) {
+        // keep the base pointer reachable for delete_forward_matrix
+        fprintf(stderr, "argweaver_amd: forward table must start at coordinate 0\n");
+        delete[] table;
+        delete[] fw;
+        return nullptr;
+    }
+    return fw;
+}
+
+void delete_forward_matrix(double **mat, int nrows)
+{
+    (void)nrows;
+    if (!mat) return;
+    delete[] mat[0];
+    delete[] mat;
+}
+
+void delete_double_matrix(double **mat, int nrows)
+{
+    (void)nrows;
+    if (!mat) return;
+    delete[] mat[0];
+    delete[] mat;
+}
+
+intstate *arghmm_sample_posterior(int **ptrees, int **ages, int **sprs, int *blocklens, int ntrees,
+                                  int nnodes, double *times, int ntimes, double *popsizes,
+                                  double rho, double mu, char **seqs, int nseqs, int seqlen,
+                                  intstate *path)
+{
+    (void)nseqs;
+    require_gpu("arghmm_sample_posterior");
+    ArgTrees *tr = trees_from_arrays(ptrees, ages, sprs, blocklens, ntrees, nnodes, 0);
+    HostInstance hi;
+    build_instance(*tr, times, ntimes, popsizes, rho, mu, seqs, tr->num_leaves(), hi);
+    std::vector<double> u(seqlen);
+    for (int i = 0; i < seqlen; i++) u[i] = frand();
+    std::vector<int> ipath(seqlen);
+    if (run_instance(hi, u.data(), ipath.data(), nullptr, nullptr, nullptr)) {
+        delete tr;
+        return nullptr;
+    }
+    if (!path) path = new intstate[seqlen];
+    // The reference converts with a shadowed block end (sample_thread.cpp:959-970):
+    // every block writes the columns starting at start_coord.  Kept as is.
+    std::vector<int> sn, st;
+    for (const LBlock &B : tr->blocks) {
+        tree_states(B.tree, ntimes, sn, st);
+        for (int i = tr->start_coord; i < tr->start_coord + B.blocklen; i++) {
+            int istate = ipath[i];
+            path[i][0] = sn[istate];
+            path[i][1] = st[istate];
+        }
+    }
+    delete tr;
+    return path;
+}
+
+LocalTrees *arghmm_sample_thread(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                                 double rho, double mu, char **seqs, int nseqs, int seqlen)
+{
+    (void)seqlen;
+    require_gpu("arghmm_sample_thread");
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, nseqs - 1)) abort();
+    return trees;
+}
+
+// sample_arg_seq (sample_arg.cpp:26-67), in input order
+static void sample_arg_seq(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
+                           double mu, char **seqs, int nseqs, int seqlen)
+{
+    if (tr.num_leaves() == 0 || tr.blocks.empty()) {
+        int start = tr.start_coord, end = tr.end_coord;
+        if (end != seqlen) {
+            start = 0;
+            end = seqlen;
+        }
+        make_trunk(tr, start, end, 2 * nseqs - 1);
+    }
+    std::vector<char> has(nseqs, 0);
+    for (int i = 0; i < tr.num_leaves(); i++) has[tr.seqids[i]] = 1;
+    for (int i = 0; i < nseqs; i++)
+        if (!has[i] && sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, i)) abort();
+}
+
+// resample_arg (sample_arg.cpp:71-107)
+static void resample_arg(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
+                         double mu, char **seqs, int nremove)
+{
+    const int nleaves = tr.num_leaves();
+    if (nremove == 1) {
+        for (int chrom = 0; chrom < nleaves; chrom++) {
+            remove_arg_thread(tr, chrom);
+            if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, chrom)) abort();
+        }
+        return;
+    }
+    if (nremove <= 0) return;
+    if (nremove > nleaves - 1) nremove = nleaves - 1;
+    std::vector<int> avail(nleaves);
+    for (int i = 0; i < nleaves; i++) avail[i] = i;
+    // shuffle with irand (common.h:109-140)
+    for (int i = 1; i < nleaves; i++) {
+        int j = int(rand() / float(RAND_MAX) * i);
+        if (j == i) j = i - 1;
+        std::swap(avail[i], avail[j]);
+    }
+    for (int i = 0; i < nremove; i++) remove_arg_thread(tr, avail[i]);
+    for (int i = 0; i < nremove; i++)
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, avail[i])) abort();
+}
+
+LocalTrees *arghmm_sample_arg_seq(double *times, int ntimes, double *popsizes, double rho, double mu,
+                                  char **seqs, int nseqs, int seqlen)
+{
+    require_gpu("arghmm_sample_arg_seq");
+    ArgTrees *tr = new ArgTrees();
+    sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen);
+    return reinterpret_cast<LocalTrees *>(tr);
+}
+
+LocalTrees *arghmm_sample_arg_refine(double *times, int ntimes, double *popsizes, double rho,
+                                     double mu, char **seqs, int nseqs, int seqlen, int niters,
+                                     int nremove)
+{
+    require_gpu("arghmm_sample_arg_refine");
+    ArgTrees *tr = new ArgTrees();
+    sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen);
+    for (int i = 0; i < niters; i++) resample_arg(*tr, times, ntimes, popsizes, rho, mu, seqs, nremove);
+    return reinterpret_cast<LocalTrees *>(tr);
+}
+
+LocalTrees *arghmm_resample_arg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                                double rho, double mu, char **seqs, int nseqs, int seqlen,
+                                int niters, int nremove)
+{
+    (void)seqlen;
+    require_gpu("arghmm_resample_arg");
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    // arghmm_complete_arg (sample_arg.cpp:655-662)
+    for (int c = tr.num_leaves(); c < nseqs; c++)
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) abort();
+    for (int i = 0; i < niters; i++) resample_arg(tr, times, ntimes, popsizes, rho, mu, seqs, nremove);
+    return trees;
+}
+
+void delete_path(int *path) { delete[] path; }
+
+void setLogLevel(int level) { g_log_level = level; }
+
+//---------------------------------------------------------------------------
+// extensions of this library (no counterpart in the reference)
+
+LocalTrees *awamd_remove_thread(LocalTrees *trees, int seqid)
+{
+    remove_arg_thread(*reinterpret_cast<ArgTrees *>(trees), seqid);
+    return trees;
+}
+
+int awamd_thread_forward(LocalTrees *trees, const double *times, int ntimes,
+                         const double *popsizes, double rho, double mu, char **seqs, int new_chrom,
+                         const double *uniforms, int precision, double *alpha_out, int *path_out,
+                         double *lnl_out, double *ms_out)
+{
+    if (!device_available()) {
+        fprintf(stderr, "argweaver_amd: awamd_thread_forward needs a HIP device\n");
+        return -1;
+    }
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    HostInstance hi;
+    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    RunStats st;
+    int rc = run_instance(hi, uniforms, path_out, alpha_out, lnl_out, &st,
+                          precision == 1 ? Precision::FP32 : Precision::FP64);
+    if (ms_out) {
+        ms_out[0] = st.ms_prep;
+        ms_out[1] = st.ms_emit;
+        ms_out[2] = st.ms_forward;
+        ms_out[3] = st.ms_traceback;
+    }
+    return rc;
+}
+
+long long awamd_num_column_states(LocalTrees *trees, int ntimes, int *nstates_per_block)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    FlatTrees ft;
+    flatten(tr, ft);
+    std::vector<int> Sv;
+    long long W = instance_states(ft, ntimes, &Sv);
+    if (nstates_per_block)
+        for (size_t i = 0; i < Sv.size(); i++) nstates_per_block[i] = Sv[i];
+    return W;
+}
+
+int awamd_get_mappings(LocalTrees *trees, int *mappings)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    const int N = tr.nnodes;
+    for (size_t b = 0; b < tr.blocks.size(); b++)
+        for (int i = 0; i < N; i++)
+            mappings[b * N + i] = tr.blocks[b].mapping.empty() ? -1 : tr.blocks[b].mapping[i];
+    return 0;
+}
+
+int awamd_get_seqids(LocalTrees *trees, int *seqids)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    for (size_t i = 0; i < tr.seqids.size(); i++) seqids[i] = tr.seqids[i];
+    return (int)tr.seqids.size();
+}
+
+int awamd_device_available(void) { return device_available() ? 1 : 0; }
+
+LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int seqid,
+                             int nrecomb, const int *recomb_pos, const int *recomb_node,
+                             const int *recomb_time)
+{
+    std::vector<int> rp(recomb_pos, recomb_pos + nrecomb), rn(recomb_node, recomb_node + nrecomb),
+        rt(recomb_time, recomb_time + nrecomb);
+    add_arg_thread(*reinterpret_cast<ArgTrees *>(trees), ntimes, path, seqid, rp, rn, rt);
+    return trees;
+}
+
+int awamd_sample_recombinations(LocalTrees *trees, const double *times, int ntimes,
+                                const double *popsizes, double rho, double mu, const int *path,
+                                int maxrecomb, int *recomb_pos, int *recomb_node, int *recomb_time)
+{
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    Model m;
+    std::vector<double> pop(ntimes);
+    for (int i = 0; i < ntimes; i++) pop[i] = popsizes[i];
+    model_init(m, ntimes, times, pop.data(), rho, mu);
+    std::vector<int> rp, rn, rt;
+    sample_recombinations(tr, m, path, rp, rn, rt);
+    for (size_t i = 0; i < rp.size() && (int)i < maxrecomb; i++) {
+        recomb_pos[i] = rp[i];
+        recomb_node[i] = rn[i];
+        recomb_time[i] = rt[i];
+    }
+    return (int)rp.size();
+}
+
+}  // extern "C"
+
+namespace aw {
+ArgTrees *simulate_arg(int k, int seqlen, const double *times, int T, const double *popsizes,
+                       double rho, double mu, uint64_t seed, char *seqs);
+}
+
+extern "C" LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
+                                      const double *popsizes, double rho, double mu,
+                                      unsigned long long seed, char *seqs)
+{
+    return reinterpret_cast<LocalTrees *>(
+        aw::simulate_arg(k, seqlen, times, ntimes, popsizes, rho, mu, seed, seqs));
+}

@@ -1,0 +1,603 @@
+// local_trees.cpp -- ARG data model and leaf-thread surgery (host).
+#include "local_trees.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
+namespace aw {
+
+void tree_from_parents(LTree &t, const int *ptree, const int *ages, int nnodes, int capacity)
+{
+    t.nnodes = nnodes;
+    t.nodes.assign(std::max(capacity, nnodes), LNode{-1, {-1, -1}, 0});
+    t.root = -1;
+    for (int i = 0; i < nnodes; i++) {
+        t.nodes[i].parent = ptree[i];
+        t.nodes[i].child[0] = t.nodes[i].child[1] = -1;
+        t.nodes[i].age = ages ? ages[i] : 0;
+    }
+    for (int i = 0; i < nnodes; i++) {
+        int p = ptree[i];
+        if (p != -1) {
+            int *c = t.nodes[p].child;
+            if (c[0] == -1) c[0] = i;
+            else c[1] = i;
+        } else {
+            t.root = i;
+        }
+    }
+}
+
+ArgTrees *trees_from_arrays(int **ptrees, int **ages, int **sprs, const int *blocklens,
+                            int ntrees, int nnodes, int start_coord)
+{
+    ArgTrees *tr = new ArgTrees();
+    tr->start_coord = start_coord;
+    tr->end_coord = start_coord;
+    tr->nnodes = nnodes;
+    tr->blocks.resize(ntrees);
+    for (int i = 0; i < ntrees; i++) {
+        LBlock &B = tr->blocks[i];
+        tree_from_parents(B.tree, ptrees[i], ages[i], nnodes, nnodes);
+        B.spr = Spr{sprs[i][0], sprs[i][1], sprs[i][2], sprs[i][3]};
+        B.blocklen = blocklens[i];
+        if (i > 0) {
+            // make_node_mapping (local_tree.h:767-776)
+            B.mapping.resize(nnodes);
+            for (int j = 0; j < nnodes; j++) B.mapping[j] = j;
+            B.mapping[ptrees[i - 1][sprs[i][0]]] = -1;
+        }
+        tr->end_coord += blocklens[i];
+    }
+    const int nleaves = tr->num_leaves();
+    tr->seqids.resize(nleaves);
+    for (int i = 0; i < nleaves; i++) tr->seqids[i] = i;
+    return tr;
+}
+
+void make_trunk(ArgTrees &tr, int start, int end, int capacity)
+{
+    tr.blocks.clear();
+    tr.start_coord = start;
+    tr.end_coord = end;
+    tr.nnodes = 1;
+    LBlock B;
+    int p = -1, a = 0;
+    tree_from_parents(B.tree, &p, &a, 1, std::max(capacity, 1));
+    B.spr = Spr{-1, -1, -1, -1};
+    B.blocklen = end - start;
+    tr.blocks.push_back(std::move(B));
+    tr.seqids.assign(1, 0);
+}
+
+// apply_spr (local_tree.cpp:223-314): the broken node is reused as the
+// re-coalescence node.
+void apply_spr(LTree &t, const Spr &spr)
+{
+    LNode *nodes = t.nodes.data();
+    if (spr.rn == t.root) return;
+
+    int recoal = nodes[spr.rn].parent;
+    int *c = nodes[recoal].child;
+    int other = (c[0] == spr.rn ? 1 : 0);
+    int recomb_sib = c[other];
+    int broke_parent = nodes[recoal].parent;
+
+    nodes[recomb_sib].parent = broke_parent;
+    int x = 0;
+    if (broke_parent != -1) {
+        c = nodes[broke_parent].child;
+        x = (c[0] == recoal ? 0 : 1);
+        nodes[broke_parent].child[x] = recomb_sib;
+    }
+
+    if (spr.cn == recoal) {
+        nodes[recoal].child[other] = recomb_sib;
+        nodes[recoal].parent = nodes[recomb_sib].parent;
+        nodes[recomb_sib].parent = recoal;
+        if (broke_parent != -1) nodes[broke_parent].child[x] = recoal;
+    } else {
+        nodes[recoal].child[other] = spr.cn;
+        nodes[recoal].parent = nodes[spr.cn].parent;
+        nodes[spr.cn].parent = recoal;
+        int parent = nodes[recoal].parent;
+        if (parent != -1) {
+            c = nodes[parent].child;
+            if (c[0] == spr.cn) c[0] = recoal;
+            else c[1] = recoal;
+        }
+    }
+    nodes[recoal].age = spr.ct;
+
+    int root;
+    if (spr.cn == t.root) root = recoal;
+    else if (recoal == t.root) root = (spr.cn == recomb_sib) ? recoal : recomb_sib;
+    else root = t.root;
+    t.root = root;
+}
+
+// rename_node (thread.cpp:9-38)
+static void rename_node(LTree &t, int src, int dest)
+{
+    if (src == dest) return;
+    LNode *nodes = t.nodes.data();
+    nodes[dest] = nodes[src];
+    int parent = nodes[dest].parent;
+    if (parent != -1) {
+        int *c = nodes[parent].child;
+        if (c[0] == src) c[0] = dest;
+        else c[1] = dest;
+    }
+    int *c = nodes[dest].child;
+    if (c[0] != -1) nodes[c[0]].parent = dest;
+    if (c[1] != -1) nodes[c[1]].parent = dest;
+}
+
+// add_tree_branch (thread.cpp:47-101): new leaf = nleaves, displaced node =
+// nnodes, new coalescence node = nnodes+1.
+void add_tree_branch(LTree &t, int node, int time)
+{
+    LNode *nodes = t.nodes.data();
+    int nleaves = t.num_leaves();
+    int nnodes = t.nnodes;
+    int newleaf = nleaves, displaced = nnodes, newcoal = nnodes + 1;
+    int node2 = (node != newleaf ? node : displaced);
+    int parent = nodes[node].parent;
+    int parent2 = (parent != newleaf ? parent : displaced);
+
+    if (newleaf < displaced) rename_node(t, newleaf, displaced);
+
+    nodes[newleaf].parent = newcoal;
+    nodes[newleaf].child[0] = nodes[newleaf].child[1] = -1;
+    nodes[newleaf].age = 0;
+
+    nodes[newcoal].parent = parent2;
+    nodes[newcoal].child[0] = newleaf;
+    nodes[newcoal].child[1] = node2;
+    nodes[newcoal].age = time;
+
+    nodes[node2].parent = newcoal;
+    if (parent2 != -1) {
+        int *c = nodes[parent2].child;
+        if (c[0] == node2) c[0] = newcoal;
+        else c[1] = newcoal;
+    }
+
+    t.nnodes = nnodes + 2;
+    if (nodes[newcoal].parent == -1) t.root = newcoal;
+    else t.root = (t.root != newleaf ? t.root : displaced);
+}
+
+// remove_tree_branch (thread.cpp:107-166)
+void remove_tree_branch(LTree &t, int remove_leaf, int *displace)
+{
+    LNode *nodes = t.nodes.data();
+    int nnodes = t.nnodes;
+    int last_leaf = t.num_leaves() - 1;
+
+    int remove_coal = nodes[remove_leaf].parent;
+    int *c = nodes[remove_coal].child;
+    int coal_child = (c[0] == remove_leaf ? c[1] : c[0]);
+    int coal_parent = nodes[remove_coal].parent;
+    nodes[coal_child].parent = coal_parent;
+    if (coal_parent != -1) {
+        c = nodes[coal_parent].child;
+        if (c[0] == remove_coal) c[0] = coal_child;
+        else c[1] = coal_child;
+    }
+
+    if (displace) {
+        for (int i = 0; i < nnodes; i++) displace[i] = i;
+        displace[remove_leaf] = -1;
+        displace[remove_coal] = -1;
+    }
+
+    if (last_leaf != remove_leaf) {
+        if (displace) displace[last_leaf] = remove_leaf;
+        rename_node(t, last_leaf, remove_leaf);
+    }
+
+    int hole = last_leaf;
+    if (remove_coal != nnodes - 2) {
+        if (displace) displace[nnodes - 2] = hole;
+        rename_node(t, nnodes - 2, hole);
+        hole = remove_coal;
+    }
+    if (remove_coal != nnodes - 1) {
+        if (displace) displace[nnodes - 1] = hole;
+        rename_node(t, nnodes - 1, hole);
+    }
+
+    t.nnodes -= 2;
+    int root = t.root;
+    if (t.root == remove_coal) root = coal_child;
+    if (root == nnodes - 2) root = last_leaf;
+    if (root == nnodes - 1) root = hole;
+    t.root = root;
+}
+
+// add_spr_branch (thread.cpp:176-266)
+static void add_spr_branch(const LTree &t, const LTree &last, int st_node, int st_time,
+                           int last_node, Spr &spr, int *mapping, int newleaf, int displaced,
+                           int newcoal)
+{
+    const LNode *nodes = t.nodes.data();
+    const LNode *last_nodes = last.nodes.data();
+
+    int last_node2 = (last_node != newleaf ? last_node : displaced);
+    int node2 = (st_node != newleaf ? st_node : displaced);
+
+    mapping[displaced] = mapping[newleaf];
+    mapping[newleaf] = newleaf;
+    mapping[newcoal] = newcoal;
+    for (int i = newleaf + 1; i < t.nnodes; i++)
+        if (mapping[i] == newleaf) mapping[i] = displaced;
+
+    if (spr.rn == newleaf) spr.rn = displaced;
+    if (spr.cn == newleaf) spr.cn = displaced;
+
+    int recoal = nodes[mapping[spr.rn]].parent;
+    if (recoal == newcoal) {
+        if (mapping[last_node2] == node2) {
+            spr.rn = newcoal;
+        } else {
+            int state_node = (st_node != newleaf) ? st_node : displaced;
+            if (state_node == mapping[spr.rn]) {
+                spr.cn = newleaf;
+                spr.ct = st_time;
+            }
+        }
+    } else {
+        const int *c = nodes[recoal].child;
+        if (c[0] == newcoal || c[1] == newcoal) {
+            if (last_nodes[spr.cn].parent == newcoal) spr.cn = newcoal;
+        }
+    }
+
+    if (last_nodes[spr.rn].parent == newcoal) {
+        mapping[newcoal] = -1;
+        int p = last_nodes[newcoal].parent;
+        if (p != -1) mapping[p] = newcoal;
+    } else {
+        int x = newcoal;
+        while (true) {
+            int y = last_nodes[x].child[0];
+            if (y == spr.cn || y == spr.rn) y = last_nodes[x].child[1];
+            x = y;
+            if (mapping[x] != -1) break;
+        }
+        mapping[newcoal] = nodes[mapping[x]].parent;
+    }
+}
+
+void tree_states(const LTree &t, int ntimes, std::vector<int> &node, std::vector<int> &time)
+{
+    node.clear();
+    time.clear();
+    for (int i = 0; i < t.nnodes; i++) {
+        int p = t.nodes[i].parent;
+        int top = (p == -1) ? ntimes - 2 : t.nodes[p].age;
+        for (int a = t.nodes[i].age; a <= top; a++) {
+            node.push_back(i);
+            time.push_back(a);
+        }
+    }
+}
+
+// add_arg_thread (thread.cpp:272-418)
+void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
+                    const std::vector<int> &recomb_pos, const std::vector<int> &recomb_node,
+                    const std::vector<int> &recomb_time)
+{
+    size_t irecomb = 0;
+    const int nleaves = tr.num_leaves();
+    const int nnodes = tr.nnodes;
+    const int nnodes2 = nnodes + 2;
+    const int newleaf = nleaves, displaced = nnodes, newcoal = nnodes + 1;
+    const int sc = tr.start_coord;
+
+    tr.seqids.push_back(seqid);
+    tr.nnodes = nnodes2;
+
+    std::vector<int> snode, stime;
+    std::vector<LBlock> out;
+    out.reserve(tr.blocks.size() + recomb_pos.size());
+    const LTree *last_tree = nullptr;
+    int last_node = 0;   // node of the state at the end of the previous block
+    int end = sc;
+    for (size_t bi = 0; bi < tr.blocks.size(); bi++) {
+        out.push_back(std::move(tr.blocks[bi]));
+        LBlock *blk = &out.back();
+        int start = end;
+        end += blk->blocklen;
+        tree_states(blk->tree, ntimes, snode, stime);
+
+        blk->tree.ensure_capacity(nnodes2);
+        if (!blk->mapping.empty()) blk->mapping.resize(std::max<size_t>(blk->mapping.size(), nnodes2));
+        int s0 = path[start - sc];
+        add_tree_branch(blk->tree, snode[s0], stime[s0]);
+        if (!blk->mapping.empty())
+            add_spr_branch(blk->tree, *last_tree, snode[s0], stime[s0], last_node, blk->spr,
+                           blk->mapping.data(), newleaf, displaced, newcoal);
+
+        for (; irecomb < recomb_pos.size() && recomb_pos[irecomb] < end; irecomb++) {
+            int pos = recomb_pos[irecomb];
+            const LTree &tree = blk->tree;
+            int sn = snode[path[pos - sc]], stt = stime[path[pos - sc]];
+            int ln = snode[path[pos - 1 - sc]], lt = stime[path[pos - 1 - sc]];
+
+            Spr spr2;
+            spr2.rn = recomb_node[irecomb];
+            spr2.rt = recomb_time[irecomb];
+            if (spr2.rn == newleaf) spr2.rn = displaced;
+            if (spr2.rn == -1) {
+                spr2.rn = newleaf;
+                spr2.cn = sn;
+                if (spr2.cn == newleaf) spr2.cn = displaced;
+                if (sn == ln && stt > lt) spr2.cn = newcoal;
+            } else {
+                if (stt > lt) spr2.cn = tree.nodes[newleaf].parent;
+                else spr2.cn = newleaf;
+            }
+            spr2.ct = stt;
+
+            LBlock nb;
+            nb.mapping.resize(tree.nodes.size());
+            for (int j = 0; j < nnodes2; j++) nb.mapping[j] = j;
+            nb.mapping[tree.nodes[spr2.rn].parent] = -1;
+            nb.tree = tree;
+            nb.tree.nnodes = nnodes2;
+            apply_spr(nb.tree, spr2);
+            nb.spr = spr2;
+
+            int block_end;
+            if (irecomb < recomb_pos.size() - 1) block_end = std::min(recomb_pos[irecomb + 1], end);
+            else block_end = end;
+            blk->blocklen = pos - start;
+            nb.blocklen = block_end - pos;
+            out.push_back(std::move(nb));
+            blk = &out.back();
+            start = pos;
+        }
+
+        last_tree = nullptr;
+        // out may reallocate on the next push_back; reserve() above prevents it
+        last_tree = &out.back().tree;
+        int le = path[end - 1 - sc];
+        last_node = snode[le];
+        if (last_node == newleaf) last_node = displaced;
+    }
+    tr.blocks = std::move(out);
+    if (!check_trees(tr)) {
+        fprintf(stderr, "argweaver_amd: add_arg_thread produced an invalid ARG\n");
+        abort();
+    }
+}
+
+// remove_null_spr / remove_null_sprs (local_tree.cpp:409-469)
+static void remove_null_sprs(ArgTrees &tr)
+{
+    std::vector<LBlock> out;
+    out.reserve(tr.blocks.size());
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        LBlock &B = tr.blocks[i];
+        bool has_next = (i + 1 < tr.blocks.size());
+        if (has_next && spr_is_null(tr.blocks[i + 1].spr)) {
+            // merge B into the next block
+            LBlock &N = tr.blocks[i + 1];
+            if (B.mapping.empty()) {
+                N.mapping.clear();
+            } else {
+                int nn = N.tree.nnodes;
+                std::vector<int> m(nn);
+                for (int j = 0; j < nn; j++) m[j] = (B.mapping[j] != -1) ? N.mapping[B.mapping[j]] : -1;
+                for (int j = 0; j < nn; j++) N.mapping[j] = m[j];
+                N.spr = B.spr;
+            }
+            N.blocklen += B.blocklen;
+            continue;
+        }
+        out.push_back(std::move(B));
+    }
+    tr.blocks = std::move(out);
+}
+
+// remove_arg_thread (thread.cpp:425-554)
+void remove_arg_thread(ArgTrees &tr, int remove_seqid)
+{
+    const int nnodes = tr.nnodes;
+    const int nleaves = tr.num_leaves();
+    std::vector<int> displace(nnodes);
+    const int last_leaf = nleaves - 1;
+
+    int remove_leaf = -1;
+    for (size_t i = 0; i < tr.seqids.size(); i++)
+        if (tr.seqids[i] == remove_seqid) {
+            remove_leaf = (int)i;
+            break;
+        }
+    if (remove_leaf == -1) {
+        fprintf(stderr, "argweaver_amd: remove_arg_thread: unknown sequence %d\n", remove_seqid);
+        abort();
+    }
+
+    if (nnodes == 3) {
+        int seqid = tr.seqids[1 - remove_leaf];
+        int cap = (int)tr.blocks.front().tree.nodes.size();
+        make_trunk(tr, tr.start_coord, tr.end_coord, cap);
+        tr.seqids[0] = seqid;
+        return;
+    }
+
+    for (size_t i = 0; i < tr.blocks.size(); i++) {
+        LTree &tree = tr.blocks[i].tree;
+        LNode *nodes = tree.nodes.data();
+        int remove_coal = nodes[remove_leaf].parent;
+        int coal_time = nodes[remove_coal].age;
+        int *c = nodes[remove_coal].child;
+        int coal_child = (c[0] == remove_leaf ? c[1] : c[0]);
+
+        remove_tree_branch(tree, remove_leaf, displace.data());
+
+        std::vector<int> &mapping = tr.blocks[i].mapping;
+        if (!mapping.empty())
+            for (int j = 0; j < nnodes - 2; j++)
+                mapping[j] = (mapping[j] != -1) ? displace[mapping[j]] : -1;
+
+        if (i + 1 >= tr.blocks.size()) continue;
+        std::vector<int> &m2 = tr.blocks[i + 1].mapping;
+        if (displace[last_leaf] != -1) m2[displace[last_leaf]] = m2[last_leaf];
+        if (displace[nnodes - 2] != -1) m2[displace[nnodes - 2]] = m2[nnodes - 2];
+        if (displace[nnodes - 1] != -1) m2[displace[nnodes - 1]] = m2[nnodes - 1];
+
+        Spr &spr = tr.blocks[i + 1].spr;
+        coal_child = displace[coal_child];
+        if (spr.rn == remove_leaf) {
+            spr = Spr{-1, -1, -1, -1};
+            continue;
+        }
+        if (spr.rn == remove_coal) spr.rn = coal_child;
+        else spr.rn = displace[spr.rn];
+        if (spr.rn == coal_child && nodes[coal_child].parent == -1) {
+            spr = Spr{-1, -1, -1, -1};
+            continue;
+        }
+        if (spr.cn == remove_leaf) {
+            spr.cn = coal_child;
+            spr.ct = coal_time;
+        } else if (spr.cn == remove_coal) {
+            spr.cn = coal_child;
+        } else {
+            spr.cn = displace[spr.cn];
+        }
+        if (spr.rn == spr.cn) {
+            spr = Spr{-1, -1, -1, -1};
+            continue;
+        }
+    }
+
+    tr.seqids[remove_leaf] = tr.seqids[last_leaf];
+    tr.seqids.resize(nleaves - 1);
+    tr.nnodes -= 2;
+    remove_null_sprs(tr);
+    if (!check_trees(tr)) {
+        fprintf(stderr, "argweaver_amd: remove_arg_thread produced an invalid ARG\n");
+        abort();
+    }
+}
+
+static bool check_tree(const LTree &t)
+{
+    const LNode *nodes = t.nodes.data();
+    for (int i = 0; i < t.nnodes; i++) {
+        for (int x = 0; x < 2; x++) {
+            int c = nodes[i].child[x];
+            if (c != -1 && (c < 0 || c >= t.nnodes || nodes[c].parent != i)) return false;
+        }
+        if (nodes[i].parent == -1) {
+            if (t.root != i) return false;
+        } else if (nodes[i].parent < 0 || nodes[i].parent >= t.nnodes) {
+            return false;
+        }
+    }
+    return t.root >= 0 && nodes[t.root].parent == -1;
+}
+
+// assert_spr (local_tree.cpp:1689-1759) as a predicate
+static bool check_spr(const LTree &last, const LTree &t, const Spr &spr, const int *mapping)
+{
+    const LNode *ln = last.nodes.data();
+    const LNode *nd = t.nodes.data();
+    if (spr.rn == -1) return false;
+    if (mapping[spr.rn] == -1) return false;
+    if (spr.rt > spr.ct) return false;
+    if (ln[spr.rn].parent == -1) return false;
+    if (spr.rt > ln[ln[spr.rn].parent].age || spr.rt < ln[spr.rn].age) return false;
+    if (spr.ct < ln[spr.cn].age) return false;
+    if (ln[spr.cn].parent != -1 && spr.ct > ln[ln[spr.cn].parent].age) return false;
+    int recoal = nd[mapping[spr.rn]].parent;
+    if (recoal == -1) return false;
+    const int *c = nd[recoal].child;
+    int other = (c[0] == mapping[spr.rn] ? c[1] : c[0]);
+    if (mapping[spr.cn] != -1) {
+        if (other != mapping[spr.cn]) return false;
+    } else {
+        int broken = ln[spr.rn].parent;
+        const int *c2 = ln[broken].child;
+        int last_other = (c2[0] == spr.rn ? c2[1] : c2[0]);
+        if (mapping[last_other] == -1) return false;
+        if (nd[mapping[last_other]].parent != recoal) return false;
+    }
+    for (int i = 0; i < last.nnodes; i++) {
+        int i2 = mapping[i];
+        if (i2 != -1 && ln[i].age != nd[i2].age) return false;
+        if (ln[i].child[0] == -1 && (i2 == -1 || nd[i2].child[0] != -1)) return false;
+    }
+    return spr.rn != spr.cn;
+}
+
+bool check_trees(const ArgTrees &tr)
+{
+    if (tr.blocks.empty()) return true;
+    if (!spr_is_null(tr.blocks[0].spr) || !tr.blocks[0].mapping.empty()) return false;
+    int seqlen = 0;
+    const LTree *last = nullptr;
+    for (const LBlock &B : tr.blocks) {
+        seqlen += B.blocklen;
+        if (B.blocklen < 0 || !check_tree(B.tree)) return false;
+        if (last) {
+            if (spr_is_null(B.spr)) {
+                std::vector<char> seen(B.tree.nnodes, 0);
+                for (int i = 0; i < B.tree.nnodes; i++) {
+                    int m = B.mapping[i];
+                    if (m == -1 || seen[m]) return false;
+                    seen[m] = 1;
+                }
+            } else if (!check_spr(*last, B.tree, B.spr, B.mapping.data())) {
+                return false;
+            }
+        }
+        last = &B.tree;
+    }
+    return seqlen == tr.length();
+}
+
+void flatten(const ArgTrees &tr, FlatTrees &f)
+{
+    const int B = (int)tr.blocks.size();
+    const int N = tr.nnodes;
+    f.nblocks = B;
+    f.nnodes = N;
+    f.start_coord = tr.start_coord;
+    f.seqlen = tr.length();
+    f.parent.resize((size_t)B * N);
+    f.child0.resize((size_t)B * N);
+    f.child1.resize((size_t)B * N);
+    f.age.resize((size_t)B * N);
+    f.spr.resize((size_t)B * 4);
+    f.mapping.resize((size_t)B * N);
+    f.start.resize(B);
+    f.len.resize(B);
+    int pos = 0;
+    for (int b = 0; b < B; b++) {
+        const LBlock &blk = tr.blocks[b];
+        for (int i = 0; i < N; i++) {
+            const LNode &nd = blk.tree.nodes[i];
+            f.parent[(size_t)b * N + i] = nd.parent;
+            f.child0[(size_t)b * N + i] = nd.child[0];
+            f.child1[(size_t)b * N + i] = nd.child[1];
+            f.age[(size_t)b * N + i] = nd.age;
+            f.mapping[(size_t)b * N + i] = blk.mapping.empty() ? -1 : blk.mapping[i];
+        }
+        f.spr[4 * b + 0] = blk.spr.rn;
+        f.spr[4 * b + 1] = blk.spr.rt;
+        f.spr[4 * b + 2] = blk.spr.cn;
+        f.spr[4 * b + 3] = blk.spr.ct;
+        f.start[b] = pos;
+        f.len[b] = blk.blocklen;
+        pos += blk.blocklen;
+    }
+}
+
+}  // namespace aw

@@ -1,0 +1,154 @@
+/*
+ * argweaver_amd.h -- C ABI of libargweaver_amd.so.
+ *
+ * The arghmm_* / get_* / delete_* entry points keep the names, argument
+ * order, C types and ownership rules of the reference library's C ABI
+ * (bound from Python by argweaver/argweaverc.py:29-343 through ctypes), so
+ * a caller can load this library in place of libargweaver.so for the
+ * threading path.  Each declaration cites the reference definition it
+ * replaces (paths relative to the reference repository root).
+ *
+ * The awamd_* entry points are extensions with no counterpart in the
+ * reference: explicit uniform streams, FP32 mode, batched and served
+ * threading, and the synthetic workload generator.
+ *
+ * Errors follow the reference: invalid input aborts the process (the
+ * reference asserts), unsupported modes print to stderr and return NULL.
+ * Calls that run the HMM need a visible HIP device and abort without one.
+ * Not thread-safe; the traceback draws from the process-global rand().
+ */
+#ifndef ARGWEAVER_AMD_H
+#define ARGWEAVER_AMD_H
+
+#ifdef __cplusplus
+#include <cstddef>
+extern "C" {
+#else
+#include <stdbool.h>
+#include <stddef.h>
+#endif
+
+typedef struct LocalTrees LocalTrees;   /* opaque ARG handle */
+typedef int intstate[2];                 /* (node, time) */
+
+/*---------------------------------------------------------------------------
+ * ARG handles
+ */
+/* src/argweaver/local_tree.cpp:1817-1824 */
+LocalTrees *arghmm_new_trees(int **ptrees, int **ages, int **sprs, int *blocklens, int ntrees,
+                             int nnodes, int start_coord);
+/* src/argweaver/local_tree.cpp:1827-1832 */
+LocalTrees *arghmm_copy_trees(LocalTrees *trees);
+/* src/argweaver/local_tree.cpp:1835-1838 */
+int get_local_trees_ntrees(LocalTrees *trees);
+/* src/argweaver/local_tree.cpp:1841-1844 */
+int get_local_trees_nnodes(LocalTrees *trees);
+/* src/argweaver/local_tree.cpp:1847-1892 */
+void get_local_trees_ptrees(LocalTrees *trees, int **ptrees, int **ages, int **sprs,
+                            int *blocklens);
+/* src/argweaver/local_tree.cpp:1945-1958 */
+void get_local_trees_blocks(LocalTrees *trees, int *starts, int *ends);
+/* src/argweaver/local_tree.cpp:1934-1942 */
+void get_treelens(LocalTrees *trees, double *times, int ntimes, double *treelens);
+/* src/argweaver/local_tree.cpp:1895-1898 */
+void delete_local_trees(LocalTrees *trees);
+
+/*---------------------------------------------------------------------------
+ * state spaces
+ */
+/* src/argweaver/states.cpp:209-226 */
+void arghmm_get_nstates(LocalTrees *trees, int ntimes, bool internal, int *nstates);
+/* src/argweaver/states.cpp:230-252 */
+intstate **get_state_spaces(LocalTrees *trees, int ntimes, bool internal);
+/* src/argweaver/states.cpp:256-261 */
+void delete_state_spaces(intstate **all_states, int ntrees);
+
+/*---------------------------------------------------------------------------
+ * threading HMM
+ */
+/* src/argweaver/sample_thread.cpp:887-926.  Row-per-column table, freed with
+ * delete_forward_matrix.  internal=true is unsupported (returns NULL). */
+double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                            double rho, double mu, char **seqs, int nseqs, int seqlen,
+                            bool prior_given, double *prior, bool internal, bool slow);
+/* src/argweaver/sample_thread.cpp:1037-1042 */
+void delete_forward_matrix(double **mat, int nrows);
+/* src/argweaver/sample_thread.cpp:1032-1035 */
+void delete_double_matrix(double **mat, int nrows);
+/* src/argweaver/sample_thread.cpp:930-977 (path conversion quirk kept) */
+intstate *arghmm_sample_posterior(int **ptrees, int **ages, int **sprs, int *blocklens, int ntrees,
+                                  int nnodes, double *times, int ntimes, double *popsizes,
+                                  double rho, double mu, char **seqs, int nseqs, int seqlen,
+                                  intstate *path);
+/* src/argweaver/sample_thread.cpp:1010-1023: threads sequence nseqs-1 */
+LocalTrees *arghmm_sample_thread(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                                 double rho, double mu, char **seqs, int nseqs, int seqlen);
+/* src/argweaver/sample_arg.cpp:665-678 */
+LocalTrees *arghmm_sample_arg_seq(double *times, int ntimes, double *popsizes, double rho,
+                                  double mu, char **seqs, int nseqs, int seqlen);
+/* src/argweaver/sample_arg.cpp:682-697 */
+LocalTrees *arghmm_sample_arg_refine(double *times, int ntimes, double *popsizes, double rho,
+                                     double mu, char **seqs, int nseqs, int seqlen, int niters,
+                                     int nremove);
+/* src/argweaver/sample_arg.cpp:701-718 */
+LocalTrees *arghmm_resample_arg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                                double rho, double mu, char **seqs, int nseqs, int seqlen,
+                                int niters, int nremove);
+/* src/argweaver/sample_thread.cpp:1026-1029 */
+void delete_path(int *path);
+/* src/argweaver/logging.cpp:126-127 */
+void setLogLevel(int level);
+
+/*---------------------------------------------------------------------------
+ * extensions (no counterpart in the reference)
+ */
+/* Remove the leaf holding sequence `seqid` from the ARG (the step
+ * resample_arg performs before re-threading). */
+LocalTrees *awamd_remove_thread(LocalTrees *trees, int seqid);
+
+/* Forward (and traceback when `uniforms` is non-NULL) for sequence
+ * `new_chrom` against the ARG, with an explicit uniform stream
+ * (uniforms[0] for the last column).  precision: 0 = FP64 table,
+ * 1 = FP32 table with per-column rescaling (log-normalisers, logL and the
+ * traceback CDF stay FP64).  alpha_out: block-major table as FP64
+ * (W doubles, see awamd_num_column_states), may be NULL.  ms_out[4]
+ * receives prep/emission/forward/traceback kernel times.  Returns 0 on
+ * success. */
+int awamd_thread_forward(LocalTrees *trees, const double *times, int ntimes,
+                         const double *popsizes, double rho, double mu, char **seqs,
+                         int new_chrom, const double *uniforms, int precision,
+                         double *alpha_out, int *path_out, double *lnl_out, double *ms_out);
+
+/* Column-states W = sum_b blocklen_b * S_b and, optionally, S_b per block. */
+long long awamd_num_column_states(LocalTrees *trees, int ntimes, int *nstates_per_block);
+/* Node mapping of every block (nnodes ints per block, -1 rows for block 0). */
+int awamd_get_mappings(LocalTrees *trees, int *mappings);
+/* leaf -> sequence row; returns the number of leaves */
+int awamd_get_seqids(LocalTrees *trees, int *seqids);
+/* 1 when a HIP device is visible */
+int awamd_device_available(void);
+
+/* Host steps of a thread insertion, exposed for callers that run the HMM
+ * elsewhere: sample the recombination points of a sampled path
+ * (recomb.cpp:151-235, draws from rand(); returns the count, at most
+ * maxrecomb stored) and splice the thread into the ARG
+ * (thread.cpp:272-418).  Recombination nodes of -1 mean the new branch. */
+int awamd_sample_recombinations(LocalTrees *trees, const double *times, int ntimes,
+                                const double *popsizes, double rho, double mu, const int *path,
+                                int maxrecomb, int *recomb_pos, int *recomb_node,
+                                int *recomb_time);
+LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int seqid,
+                             int nrecomb, const int *recomb_pos, const int *recomb_node,
+                             const int *recomb_time);
+
+/* Seeded DSMC workload: k haplotypes over seqlen (compressed) columns with
+ * per-column rho and mu.  Fills seqs[k*seqlen] row-major and returns the
+ * ARG of all k leaves (NULL on failure). */
+LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
+                           const double *popsizes, double rho, double mu,
+                           unsigned long long seed, char *seqs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
