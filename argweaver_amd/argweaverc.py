@@ -113,7 +113,31 @@ awamd_simulate = _export(
     [c_int, c_int, c_double_p, c_int, c_double_p, c_double, c_double,
      c_ulonglong, ctypes.c_char_p])
 
+awamd_device_info = _export("awamd_device_info", c_int, [ctypes.c_char_p, c_int])
+awamd_batch_new = _export("awamd_batch_new", c_void_p, [])
+awamd_batch_free = _export("awamd_batch_free", None, [c_void_p])
+awamd_batch_add = _export(
+    "awamd_batch_add", c_int,
+    [c_void_p, c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p,
+     c_int, c_double_p])
+awamd_batch_upload = _export("awamd_batch_upload", c_int, [c_void_p])
+awamd_batch_run = _export("awamd_batch_run", c_int, [c_void_p, c_int, c_int, c_int, c_double_p])
+awamd_batch_run_opts = _export(
+    "awamd_batch_run_opts", c_int,
+    [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double, c_double_p])
+awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
+awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
+awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_double_p])
+awamd_batch_lnl = _export("awamd_batch_lnl", c_double, [c_void_p, c_int])
+awamd_batch_W = _export("awamd_batch_W", c_longlong, [c_void_p, c_int])
+
 _libc = ctypes.CDLL(None)
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(512)
+    awamd_device_info(buf, 512)
+    return buf.value.decode()
 
 
 def set_random_seed(num):
@@ -339,3 +363,64 @@ def get_time_points(ntimes=20, maxtime=200e3, delta=.01):
     """Time grid (argweaver/__init__.py:97-105, model.h:21-32)."""
     return [(np.exp(i / float(ntimes - 1) * np.log(1 + delta * maxtime)) - 1) / delta
             for i in range(ntimes)]
+
+
+class Batch(object):
+    """Many independent threading instances resident in HBM (awamd_batch_*)."""
+
+    STAT_NAMES = ("ms_prep", "ms_emit", "ms_forward", "ms_traceback", "ms_total", "W",
+                  "nvar", "fw_passes", "tb_passes", "fw_rerun_cols", "tb_rerun_cols")
+
+    def __init__(self):
+        self.ptr = awamd_batch_new()
+        self._keep = []
+        self.L = []
+
+    def add(self, trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=None):
+        times = np.ascontiguousarray(times, np.float64)
+        popsizes = np.ascontiguousarray(popsizes, np.float64)
+        cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+        self._keep.append((times, popsizes, cs, u))
+        self.L.append(cs.seqlen)
+        return awamd_batch_add(self.ptr, trees, _dptr(times), len(times), _dptr(popsizes), rho,
+                               mu, cs.ptr, new_chrom, _dptr(u) if u is not None else None)
+
+    def upload(self):
+        if awamd_batch_upload(self.ptr) != 0:
+            raise RuntimeError("awamd_batch_upload failed")
+        self._keep = []
+
+    def run(self, precision=0, chunk_cols=0, traceback=True, fixup=True, tol_rel=1e-6,
+            tol_abs=1e-9):
+        st = np.zeros(11, np.float64)
+        if awamd_batch_run_opts(self.ptr, precision, chunk_cols, int(traceback), int(fixup),
+                                tol_rel, tol_abs, _dptr(st)) != 0:
+            raise RuntimeError("awamd_batch_run failed")
+        return dict(zip(self.STAT_NAMES, st.tolist()))
+
+    def path(self, i):
+        out = np.zeros(self.L[i], np.int32)
+        awamd_batch_path(self.ptr, i, _iptr(out))
+        return out
+
+    def alpha(self, i):
+        out = np.zeros(int(awamd_batch_W(self.ptr, i)), np.float64)
+        awamd_batch_alpha(self.ptr, i, _dptr(out))
+        return out
+
+    def norms(self, i):
+        out = np.zeros(self.L[i], np.float64)
+        awamd_batch_norms(self.ptr, i, _dptr(out))
+        return out
+
+    def lnl(self, i):
+        return awamd_batch_lnl(self.ptr, i)
+
+    def W(self, i):
+        return int(awamd_batch_W(self.ptr, i))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            awamd_batch_free(self.ptr)
+            self.ptr = None

@@ -560,6 +560,93 @@ int awamd_get_seqids(LocalTrees *trees, int *seqids)
 
 int awamd_device_available(void) { return device_available() ? 1 : 0; }
 
+int awamd_device_info(char *buf, int n) { return device_info(buf, n); }
+
+//---------------------------------------------------------------------------
+// batches of independent threading instances
+
+struct AwBatch {
+    Batch batch;
+    std::vector<HostInstance> staged;
+};
+
+void *awamd_batch_new(void) { return new AwBatch(); }
+
+void awamd_batch_free(void *b) { delete reinterpret_cast<AwBatch *>(b); }
+
+int awamd_batch_add(void *b, LocalTrees *trees, const double *times, int ntimes,
+                    const double *popsizes, double rho, double mu, char **seqs, int new_chrom,
+                    const double *uniforms)
+{
+    AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
+    HostInstance hi;
+    build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu, seqs,
+                   new_chrom, hi);
+    return ab.batch.add(hi, uniforms);
+}
+
+int awamd_batch_upload(void *b)
+{
+    if (!device_available()) {
+        fprintf(stderr, "argweaver_amd: awamd_batch_upload needs a HIP device\n");
+        return -1;
+    }
+    return reinterpret_cast<AwBatch *>(b)->batch.upload();
+}
+
+int awamd_batch_run_opts(void *b, int precision, int chunk_cols, int traceback, int fixup,
+                         double tol_rel, double tol_abs, double *stats)
+{
+    AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
+    BatchOptions opt;
+    opt.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
+    opt.chunk_cols = chunk_cols;
+    opt.traceback = traceback != 0;
+    opt.fixup = fixup != 0;
+    opt.tol_rel = (float)tol_rel;
+    opt.tol_abs = (float)tol_abs;
+    RunStats st;
+    int rc = ab.batch.run(opt, &st);
+    if (stats) {
+        stats[0] = st.ms_prep;
+        stats[1] = st.ms_emit;
+        stats[2] = st.ms_forward;
+        stats[3] = st.ms_traceback;
+        stats[4] = st.ms_total;
+        stats[5] = (double)st.W;
+        stats[6] = st.nvar;
+        stats[7] = st.fw_passes;
+        stats[8] = st.tb_passes;
+        stats[9] = (double)st.fw_rerun_cols;
+        stats[10] = (double)st.tb_rerun_cols;
+    }
+    return rc;
+}
+
+int awamd_batch_run(void *b, int precision, int chunk_cols, int traceback, double *stats)
+{
+    return awamd_batch_run_opts(b, precision, chunk_cols, traceback, 1, 1e-6, 1e-9, stats);
+}
+
+int awamd_batch_path(void *b, int i, int *path)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.copy_path(i, path);
+}
+
+int awamd_batch_alpha(void *b, int i, double *alpha)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.copy_alpha(i, alpha);
+}
+
+int awamd_batch_norms(void *b, int i, double *norms)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.copy_norms(i, norms);
+}
+
+double awamd_batch_lnl(void *b, int i) { return reinterpret_cast<AwBatch *>(b)->batch.lnl(i); }
+
+long long awamd_batch_W(void *b, int i) { return reinterpret_cast<AwBatch *>(b)->batch.W(i); }
+
 LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int seqid,
                              int nrecomb, const int *recomb_pos, const int *recomb_node,
                              const int *recomb_time)

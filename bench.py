@@ -1,0 +1,194 @@
+"""Threading-HMM throughput on one MI355X.
+
+One step = one pass of the threading path (column classification, per-block
+transition/switch/emission preparation, variant-column emissions, forward
+and stochastic traceback) over a batch of independent threading instances
+whose inputs (ARG blocks, sequences, uniforms) are already resident in HBM.
+
+Workload (default "batched"): 256 instances, k ~ U{8..64} haplotypes,
+L log-uniform on [50 kb, 2 Mb] compressed 10 bp -> 5k..200k columns,
+ntimes=20, maxtime=200e3, N=1e4, r=1.6e-8, mu=1.8e-8 (rho, mu x 10 per
+column); seeded DSMC ARGs with one leaf removed per instance.
+
+value: column-states per second (W = sum over blocks blocklen * nstates)
+of the FP32 column-tiled path (normalised FP32 table, FP64 normalisers,
+log-likelihood and traceback CDF).  The plain FP64 path (the drop-in
+arithmetic) is timed on the same batch and reported beside it.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK = 8.0e12   # B/s, MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(A, n, seed, kmin=8, kmax=64, lmin=5000, lmax=200000):
+    import oracle_bind as O   # time grid helper only (no oracle compute)
+    rng = np.random.RandomState(seed)
+    times = O.time_points(20, 200e3)
+    pops = np.full(20, 1e4)
+    rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
+    insts = []
+    for i in range(n):
+        k = int(rng.randint(kmin, kmax + 1))
+        L = int(np.exp(rng.uniform(np.log(lmin), np.log(lmax))))
+        tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed * 100003 + i)
+        A.remove_thread(tr, k - 1)
+        u = rng.uniform(size=L)
+        insts.append(dict(k=k, L=L, trees=tr, seqs=seqs, u=u))
+    return insts, (times, pops, rho, mu)
+
+
+def cpu_baseline(A, insts, model, budget_s=15.0):
+    """Single-core CPU time of the same work on a bounded sample: the
+    reference library compiled from its sources (oracle/_ref) threading
+    instances with arghmm_sample_thread, or the oracle restatement when the
+    reference build is absent."""
+    import oracle_bind as O
+    times, pops, rho, mu = model
+    order = sorted(range(len(insts)), key=lambda i: insts[i]["L"] * insts[i]["k"])
+    W = 0
+    t_used = 0.0
+    n = 0
+    kind = "reference" if O.ref_available() else "port"
+    for i in order:
+        it = insts[i]
+        w, _ = A.num_column_states(it["trees"], 20)
+        P, Ag, S, bl = A.trees_arrays(it["trees"])
+        if kind == "reference":
+            rt = O.ref_new_trees(P, Ag, S, bl)
+            O.srand(1)
+            t0 = time.perf_counter()
+            O.ref_sample_thread(rt, it["seqs"], times, pops, rho, mu)
+            dt = time.perf_counter() - t0
+            O.ref_delete(rt)
+        else:
+            maps, seqids = A.trees_internal(it["trees"])
+            inst = O.Instance(times, pops, rho, mu, P, Ag, S, bl, it["seqs"], seqids=seqids,
+                              new_chrom=it["k"] - 1, mappings=maps)
+            t0 = time.perf_counter()
+            fw, _ = inst.forward()
+            inst.traceback(fw, it["u"])
+            dt = time.perf_counter() - t0
+        W += w
+        t_used += dt
+        n += 1
+        if t_used >= budget_s:
+            break
+    return dict(value=W / t_used, unit="column-states/s", cores=1, kind=kind,
+                sample="%d smallest instances of the workload (W=%.3g column-states, %.1f s)%s"
+                % (n, W, t_used, "; arghmm_sample_thread: forward, traceback, recombination "
+                   "sampling and add-thread" if kind == "reference" else
+                   "; oracle forward + traceback"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--chunk", type=int, default=5000, help="FP32 column tile length")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the threading instances shard across GPUs with no "
+              "exchange, so --gpus N>1 is not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import argweaver_amd.argweaverc as A
+    if not A.awamd_device_available():
+        raise SystemExit("no HIP device visible")
+    log(A.device_info())
+    t0 = time.time()
+    insts, model = make_workload(A, args.instances, args.seed)
+    log("workload: %d instances generated in %.1f s" % (len(insts), time.time() - t0))
+
+    batch = A.Batch()
+    for it in insts:
+        batch.add(it["trees"], it["seqs"], *model, it["k"] - 1, it["u"])
+    batch.upload()
+    Wtot = sum(batch.W(i) for i in range(len(insts)))
+    log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
+
+    def timed(prec, chunk):
+        for _ in range(args.warmup):
+            batch.run(prec, chunk)
+        t0 = time.perf_counter()
+        stats = [batch.run(prec, chunk) for _ in range(args.steps)]
+        wall = (time.perf_counter() - t0) / args.steps
+        return wall, stats
+
+    wall32, st32 = timed(1, args.chunk)
+    fwd_ms = float(np.mean([s["ms_forward"] for s in st32]))
+    tb_ms = float(np.mean([s["ms_traceback"] for s in st32]))
+    log("fp32 tiles: %.1f ms/step (forward %.1f, traceback %.1f, passes fw %d tb %d)"
+        % (wall32 * 1e3, fwd_ms, tb_ms, st32[-1]["fw_passes"], st32[-1]["tb_passes"]))
+    lnl32 = [batch.lnl(i) for i in range(len(insts))]
+
+    extra = {}
+    if not args.no_fp64:
+        wall64, st64 = timed(0, 0)
+        lnl64 = [batch.lnl(i) for i in range(len(insts))]
+        rel = max(abs(a - b) / abs(b) for a, b in zip(lnl32, lnl64))
+        extra["fp64_plain"] = dict(value=Wtot / wall64, ms_per_step=wall64 * 1e3,
+                                   ms_forward=float(np.mean([s["ms_forward"] for s in st64])),
+                                   ms_traceback=float(np.mean([s["ms_traceback"] for s in st64])))
+        extra["fp32_vs_fp64"] = (Wtot / wall32) / (Wtot / wall64)
+        extra["max_rel_dlnl_fp32_vs_fp64"] = rel
+        log("fp64 plain: %.1f ms/step, max |dlnl|/|lnl| fp32 vs fp64 %.2e" % (wall64 * 1e3, rel))
+
+    # dominant kernel: the FP32 forward; algorithmic bytes = one FP32 write
+    # per column-state (the traceback reads it back)
+    achieved = (Wtot * 4) / (fwd_ms * 1e-3)
+    roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK, traffic=None,
+                    kernel="k_forward32", algorithmic_bytes_per_launch=Wtot * 4,
+                    ms_per_launch_sum=fwd_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(A, insts, model, args.cpu_budget)
+        log("cpu baseline: %.3g column-states/s (%s)" % (cpu["value"], cpu["kind"]))
+
+    out = {
+        "metric": "threaded columns*states/s (fwd+traceback) FP32-batched vs FP64; p99 latency",
+        "value": Wtot / wall32,
+        "unit": "column-states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 table, fp64 normalisers/CDF",
+        "data": "synthetic (seeded DSMC ARGs, one leaf removed per instance)",
+        "config": {"workload": "batched: %d instances, k~U{8..64}, L log-uniform 50kb-2Mb at c=10, "
+                               "ntimes=20" % len(insts),
+                   "instances": len(insts), "column_states": Wtot, "chunk_cols": args.chunk},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

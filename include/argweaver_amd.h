@@ -127,6 +127,8 @@ int awamd_get_mappings(LocalTrees *trees, int *mappings);
 int awamd_get_seqids(LocalTrees *trees, int *seqids);
 /* 1 when a HIP device is visible */
 int awamd_device_available(void);
+/* one-line description of the current device (name, CUs, LDS sizes) */
+int awamd_device_info(char *buf, int n);
 
 /* Host steps of a thread insertion, exposed for callers that run the HMM
  * elsewhere: sample the recombination points of a sampled path
@@ -140,6 +142,31 @@ int awamd_sample_recombinations(LocalTrees *trees, const double *times, int ntim
 LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int seqid,
                              int nrecomb, const int *recomb_pos, const int *recomb_node,
                              const int *recomb_time);
+
+/* Batches of independent threading instances.  awamd_batch_add stages an
+ * instance (the ARG is flattened at once; sequence rows and uniforms must
+ * stay valid until awamd_batch_upload, which copies every input to HBM).
+ * awamd_batch_run executes classification, block preparation, emissions,
+ * forward and traceback for all instances.  precision 0: plain FP64 path,
+ * one workgroup per instance.  precision 1: FP32 column tiles of
+ * chunk_cols columns (0: one tile per instance) with fix-up passes.
+ * stats[11]: prep, emit, forward, traceback and total ms, W, variant
+ * columns, forward/traceback fix-up passes and re-run columns. */
+void *awamd_batch_new(void);
+void awamd_batch_free(void *batch);
+int awamd_batch_add(void *batch, LocalTrees *trees, const double *times, int ntimes,
+                    const double *popsizes, double rho, double mu, char **seqs, int new_chrom,
+                    const double *uniforms);
+int awamd_batch_upload(void *batch);
+int awamd_batch_run(void *batch, int precision, int chunk_cols, int traceback, double *stats);
+/* as awamd_batch_run, with the FP32 tile fix-up switch and agreement tolerance */
+int awamd_batch_run_opts(void *batch, int precision, int chunk_cols, int traceback, int fixup,
+                         double tol_rel, double tol_abs, double *stats);
+int awamd_batch_path(void *batch, int i, int *path);
+int awamd_batch_alpha(void *batch, int i, double *alpha);
+int awamd_batch_norms(void *batch, int i, double *norms);
+double awamd_batch_lnl(void *batch, int i);
+long long awamd_batch_W(void *batch, int i);
 
 /* Seeded DSMC workload: k haplotypes over seqlen (compressed) columns with
  * per-column rho and mu.  Fills seqs[k*seqlen] row-major and returns the
