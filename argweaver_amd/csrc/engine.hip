@@ -889,12 +889,15 @@ __device__ inline void lane32_setup(const DevInst &d, int b, Lane32<RM> &ls, flo
     const int lane = threadIdx.x;
     const int T1 = d.T - 1, TT = T1 * T1, N = d.N;
     const int so = cp(d.soff)[b];
+    // the instance's slot table has d.RM <= RM slots per lane (the kernel is
+    // instantiated for the largest RM of the batch); the rest stay empty
+    const int IR = d.RM;
     ls.key = gp(d.lkey)[(size_t)b * 64 + lane];
     int knext = __shfl_down(ls.key, 1, 64);
     ls.gend = (ls.key < T1) && (lane == 63 || knext != ls.key);
 #pragma unroll
     for (int r = 0; r < RM; r++) {
-        int k = gp(d.lslot)[((size_t)b * 64 + lane) * RM + r];
+        int k = (r < IR) ? gp(d.lslot)[((size_t)b * 64 + lane) * IR + r] : -1;
         ls.slot[r] = k;
         ls.j1[r] = 0; ls.ln[r] = 0; ls.cb[r] = 0; ls.cs[r] = 1; ls.ei[r] = 0.f; ls.u[r] = 0.f;
         if (k >= 0) {

@@ -46,9 +46,10 @@ def make_workload(A, n, seed, kmin=8, kmax=64, lmin=5000, lmax=200000):
         k = int(rng.randint(kmin, kmax + 1))
         L = int(np.exp(rng.uniform(np.log(lmin), np.log(lmax))))
         tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed * 100003 + i)
+        full = A.trees_arrays(tr)   # the ARG before removal, for the CPU baseline
         A.remove_thread(tr, k - 1)
         u = rng.uniform(size=L)
-        insts.append(dict(k=k, L=L, trees=tr, seqs=seqs, u=u))
+        insts.append(dict(k=k, L=L, trees=tr, full=full, seqs=seqs, u=u))
     return insts, (times, pops, rho, mu)
 
 
@@ -67,15 +68,19 @@ def cpu_baseline(A, insts, model, budget_s=15.0):
     for i in order:
         it = insts[i]
         w, _ = A.num_column_states(it["trees"], 20)
-        P, Ag, S, bl = A.trees_arrays(it["trees"])
         if kind == "reference":
-            rt = O.ref_new_trees(P, Ag, S, bl)
+            # the array form of an ARG implies identity node mappings between
+            # blocks, which only holds before a thread is removed: rebuild the
+            # full ARG and remove the leaf with the reference's own routine
+            rt = O.ref_new_trees(*it["full"])
+            O.ref_remove_thread(rt, it["k"] - 1)
             O.srand(1)
             t0 = time.perf_counter()
             O.ref_sample_thread(rt, it["seqs"], times, pops, rho, mu)
             dt = time.perf_counter() - t0
             O.ref_delete(rt)
         else:
+            P, Ag, S, bl = A.trees_arrays(it["trees"])
             maps, seqids = A.trees_internal(it["trees"])
             inst = O.Instance(times, pops, rho, mu, P, Ag, S, bl, it["seqs"], seqids=seqids,
                               new_chrom=it["k"] - 1, mappings=maps)
@@ -103,6 +108,10 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=5000, help="FP32 column tile length")
+    ap.add_argument("--tol-rel", type=float, default=1e-4,
+                    help="fix-up tolerance of a re-run FP32 tile column (relative)")
+    ap.add_argument("--tol-abs", type=float, default=1e-7,
+                    help="fix-up tolerance of a re-run FP32 tile column (absolute)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -112,8 +121,9 @@ def main():
               "exchange, so --gpus N>1 is not run here", file=sys.stderr)
         sys.exit(2)
 
+    import argweaver_amd
     import argweaver_amd.argweaverc as A
-    if not A.awamd_device_available():
+    if not argweaver_amd.device_available():
         raise SystemExit("no HIP device visible")
     log(A.device_info())
     t0 = time.time()
@@ -128,10 +138,11 @@ def main():
     log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
 
     def timed(prec, chunk):
+        kw = dict(tol_rel=args.tol_rel, tol_abs=args.tol_abs)
         for _ in range(args.warmup):
-            batch.run(prec, chunk)
+            batch.run(prec, chunk, **kw)
         t0 = time.perf_counter()
-        stats = [batch.run(prec, chunk) for _ in range(args.steps)]
+        stats = [batch.run(prec, chunk, **kw) for _ in range(args.steps)]
         wall = (time.perf_counter() - t0) / args.steps
         return wall, stats
 

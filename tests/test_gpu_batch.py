@@ -1,0 +1,107 @@
+"""Batched engine on the GPU: mixed-size batches in FP64 (against the oracle)
+and FP32 (exact and column-tiled, against FP64 and the oracle), and the
+same instance alone versus inside a mix."""
+import numpy as np
+import pytest
+
+from workloads import instance
+
+pytestmark = pytest.mark.gpu
+
+# k=64 instances need 16 slots per lane in the FP32 schedule, k<=40 need 8:
+# the mix exercises per-instance slot tables under one kernel instantiation
+MIX = [(4, 700, 21), (64, 2500, 22), (12, 3000, 23), (40, 1800, 24), (8, 1200, 25)]
+
+
+def _build(aw, cases):
+    out = []
+    for k, L, seed in cases:
+        tr, seqs, inst, model = instance(aw, k, L, seed)
+        u = np.random.RandomState(seed).uniform(size=L)
+        out.append(dict(k=k, L=L, seed=seed, trees=tr, seqs=seqs, inst=inst, model=model, u=u))
+    return out
+
+
+def _batch(aw, items):
+    b = aw.Batch()
+    for it in items:
+        b.add(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+    b.upload()
+    return b
+
+
+@pytest.fixture(scope="module")
+def mix(aw):
+    items = _build(aw, MIX)
+    for it in items:
+        fw, lnl = it["inst"].forward()
+        it["fw"], it["lnl"] = fw, lnl
+        it["path"] = it["inst"].traceback(fw, it["u"])
+    return items
+
+
+def test_batch_fp64_matches_oracle(aw, mix):
+    print("batch fp64 mix", [(m["k"], m["L"]) for m in mix], flush=True)
+    b = _batch(aw, mix)
+    b.run(0, 0)
+    for i, it in enumerate(mix):
+        a = b.alpha(i)
+        rel = np.abs(a - it["fw"]) / np.maximum(np.abs(it["fw"]), 1e-300)
+        assert rel.max() <= 1e-12, (i, rel.max())
+        assert abs(b.lnl(i) - it["lnl"]) <= 1e-12 * abs(it["lnl"])
+        assert np.array_equal(b.path(i), it["path"]), i
+
+
+@pytest.mark.parametrize("chunk", [0, 300, 1000])
+def test_batch_fp32_close_to_fp64(aw, mix, chunk):
+    print("batch fp32 mix chunk=%d" % chunk, flush=True)
+    b = _batch(aw, mix)
+    st = b.run(1, chunk, tol_rel=1e-6, tol_abs=1e-9)
+    assert st["W"] == sum(len(m["fw"]) for m in mix)
+    for i, it in enumerate(mix):
+        lnl = b.lnl(i)
+        # FP32 normalised columns, FP64 normalisers: |dlogL| <= 1e-6 relative
+        assert abs(lnl - it["lnl"]) <= 1e-6 * abs(it["lnl"]), (i, lnl, it["lnl"])
+        # the FP32 table is normalised per column; compare with the oracle's
+        # columns normalised the same way
+        a32 = b.alpha(i)
+        offs = it["inst"].block_offsets()
+        nst = it["inst"].nstates()
+        fw = it["fw"]
+        err = 0.0
+        for blk in range(len(offs) - 1):
+            o0, o1, S = int(offs[blk]), int(offs[blk + 1]), int(nst[blk])
+            ref = fw[o0:o1].reshape(-1, S)
+            got = a32[o0:o1].reshape(-1, S)
+            refn = ref / ref.sum(axis=1, keepdims=True)
+            err = max(err, np.abs(got - refn).max())
+        assert err <= 2e-4, (i, err)
+        agree = np.mean(b.path(i) == it["path"])
+        assert agree >= 0.98, (i, agree)
+
+
+def test_batch_fp32_tiled_matches_untiled(aw, mix):
+    print("batch fp32 tiled vs untiled", flush=True)
+    b = _batch(aw, mix)
+    b.run(1, 0)
+    ref = [(b.lnl(i), b.path(i), b.alpha(i)) for i in range(len(mix))]
+    b.run(1, 250, tol_rel=1e-6, tol_abs=1e-9)
+    for i in range(len(mix)):
+        lnl, path, a = ref[i]
+        assert abs(b.lnl(i) - lnl) <= 1e-7 * abs(lnl), i
+        assert np.abs(b.alpha(i) - a).max() <= 1e-5, i
+        assert np.mean(b.path(i) == path) >= 0.99, i
+
+
+@pytest.mark.parametrize("prec,chunk", [(0, 0), (1, 0), (1, 400)])
+def test_alone_equals_in_mix(aw, mix, prec, chunk):
+    """An instance's results do not depend on the rest of the batch."""
+    print("alone vs mix prec=%d chunk=%d" % (prec, chunk), flush=True)
+    b = _batch(aw, mix)
+    b.run(prec, chunk)
+    for i in (0, 1, 3):
+        one = _batch(aw, [mix[i]])
+        one.run(prec, chunk)
+        assert one.lnl(0) == b.lnl(i), i
+        assert np.array_equal(one.path(0), b.path(i)), i
+        assert np.array_equal(one.alpha(0), b.alpha(i)), i
