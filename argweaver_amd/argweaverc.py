@@ -123,8 +123,7 @@ awamd_batch_add = _export(
 awamd_batch_upload = _export("awamd_batch_upload", c_int, [c_void_p])
 awamd_batch_run = _export("awamd_batch_run", c_int, [c_void_p, c_int, c_int, c_int, c_double_p])
 awamd_batch_run_opts = _export(
-    "awamd_batch_run_opts", c_int,
-    [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double, c_double_p])
+    "awamd_batch_run_opts", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p])
 awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
 awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_double_p])
@@ -369,7 +368,7 @@ class Batch(object):
     """Many independent threading instances resident in HBM (awamd_batch_*)."""
 
     STAT_NAMES = ("ms_prep", "ms_emit", "ms_forward", "ms_traceback", "ms_total", "W",
-                  "nvar", "fw_passes", "tb_passes", "fw_rerun_cols", "tb_rerun_cols")
+                  "nvar", "tb_passes", "tb_rerun_cols")
 
     def __init__(self):
         self.ptr = awamd_batch_new()
@@ -391,11 +390,12 @@ class Batch(object):
             raise RuntimeError("awamd_batch_upload failed")
         self._keep = []
 
-    def run(self, precision=0, chunk_cols=0, traceback=True, fixup=True, tol_rel=1e-6,
-            tol_abs=1e-9):
-        st = np.zeros(11, np.float64)
-        if awamd_batch_run_opts(self.ptr, precision, chunk_cols, int(traceback), int(fixup),
-                                tol_rel, tol_abs, _dptr(st)) != 0:
+    def run(self, precision=0, tile_cols=0, traceback=True, fixup=True):
+        """precision 0: FP64 table; 1: FP32 table with FP64 normalisers and
+        traceback CDF.  tile_cols: traceback tile length (0: whole instance)."""
+        st = np.zeros(len(self.STAT_NAMES), np.float64)
+        if awamd_batch_run_opts(self.ptr, precision, tile_cols, int(traceback), int(fixup),
+                                _dptr(st)) != 0:
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
 

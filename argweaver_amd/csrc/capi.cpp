@@ -594,17 +594,15 @@ int awamd_batch_upload(void *b)
     return reinterpret_cast<AwBatch *>(b)->batch.upload();
 }
 
-int awamd_batch_run_opts(void *b, int precision, int chunk_cols, int traceback, int fixup,
-                         double tol_rel, double tol_abs, double *stats)
+int awamd_batch_run_opts(void *b, int precision, int tile_cols, int traceback, int fixup,
+                         double *stats)
 {
     AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
     BatchOptions opt;
     opt.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
-    opt.chunk_cols = chunk_cols;
+    opt.tile_cols = tile_cols;
     opt.traceback = traceback != 0;
     opt.fixup = fixup != 0;
-    opt.tol_rel = (float)tol_rel;
-    opt.tol_abs = (float)tol_abs;
     RunStats st;
     int rc = ab.batch.run(opt, &st);
     if (stats) {
@@ -615,17 +613,15 @@ int awamd_batch_run_opts(void *b, int precision, int chunk_cols, int traceback, 
         stats[4] = st.ms_total;
         stats[5] = (double)st.W;
         stats[6] = st.nvar;
-        stats[7] = st.fw_passes;
-        stats[8] = st.tb_passes;
-        stats[9] = (double)st.fw_rerun_cols;
-        stats[10] = (double)st.tb_rerun_cols;
+        stats[7] = st.tb_passes;
+        stats[8] = (double)st.tb_rerun_cols;
     }
     return rc;
 }
 
-int awamd_batch_run(void *b, int precision, int chunk_cols, int traceback, double *stats)
+int awamd_batch_run(void *b, int precision, int tile_cols, int traceback, double *stats)
 {
-    return awamd_batch_run_opts(b, precision, chunk_cols, traceback, 1, 1e-6, 1e-9, stats);
+    return awamd_batch_run_opts(b, precision, tile_cols, traceback, 1, stats);
 }
 
 int awamd_batch_path(void *b, int i, int *path)

@@ -13,58 +13,80 @@ namespace aw {
 
 // Everything one threading instance keeps in HBM.  POD; an array of these
 // lives in device memory and every kernel finds its instance through it.
-// Per-block arrays are indexed through soff (states) and aoff (forward
-// table).  Switch arrays for the switch into block b are indexed by the
-// source block's state offset (determ/determp/invidx) or by the target
-// block's (recombrow/recoalrow, invptr with one extra entry per block).
+//
+// Lane-run layout.  The HMM kernels run one 64-lane wave per instance (or
+// per traceback tile); lane l owns the R consecutive states l*R .. l*R+R-1
+// of the node-major state order, so the states of one branch sit in one
+// lane or in neighbouring lanes.  Every per-state array is stored
+// transposed, position(s) = (s % R) * 64 + s / R, in rows of RW = 64*R
+// entries: slot r of all 64 lanes is one contiguous 64-entry segment, so
+// the loads and stores of a slot are coalesced.  R is fixed per instance
+// (the smallest supported value with 64*R >= max states per block), so
+// forward-table rows have a constant stride and column col starts at
+// col * RW.
 struct DevInst {
-    int T, N, B, L, nrows, Smax, TC, RM, nvar;
-    long long W;
+    int T, N, B, L, nrows, Smax, TC, R, nvar;
+    long long W;              // column-states sum_b blocklen_b * S_b
     const Model *model;
     // trees
     const int *parent, *c0, *c1, *age, *map, *spr, *bstart, *blen;
-    const int *soff;          // [B+1]
-    const long long *aoff;    // [B+1]
+    const int *soff;          // [B+1] state offsets (state-ordered arrays)
     const char *seq;          // [nrows*L], leaves then the new sequence
     const double *u;          // [L] uniforms, u[L-1-col] used at column col
-    // column classes
+    // column classes: -1 invariant, -2 masked, >=0 variant row index
     int *vflag, *cls, *vcol;  // [L]
-    // prep outputs (FP64)
+    // prep outputs in state order (pruning and traceback lookups)
     int *snode, *stime;       // [Stot]
-    int *nfirst, *nhi;        // [B*N]
-    int *tperm, *tstart;      // [Stot], [B*(T+1)]
-    double *tm, *tm2, *tm2s;  // [B*(T-1)^2], [B*TC], [B*TC]
-    double *tv;               // [B*9*T] transition vectors (D,E,...,norecombs)
-    double *einv, *eterm;     // [Stot], [Stot*6]
-    int *determ;              // [Stot]
-    double *determp;          // [Stot]
-    double *recombrow, *recoalrow;   // [Stot]
-    int *srcs;                // [2B]
-    int *invptr, *invidx;     // [Stot+B], [Stot]
-    double *prior;            // [Smax]
-    int *order;               // [B*N] postorder
-    int *roots;               // [B]
+    double *eterm;            // [Stot*6]
+    int *order, *roots;       // [B*N] postorder, [B]
     double *nmut;             // [B*N*2]
-    double *vemit;            // [nvar*Smax]
-    // FP32 path: tables and the time-grouped lane assignment of every block
-    float *tm32, *tm2_32;     // [B*(T-1)^2], [B*TC]
-    float *einv32;            // [Stot]
-    int *lslot;               // [B*64*RM] state index per (lane, slot), -1 empty
-    int *lkey;                // [B*64] time of the lane's states (T-1 when empty)
+    double *tm, *tm2s;        // [B*(T-1)^2] P(a->b) other branch, [B*TC] same branch
+    int *srcs;                // [2B] recombination / recoalescence source of the switch into b
+    // prep outputs in the lane-run layout, one row of RW per block
+    int *linfo;               // [B*RW] time | flags | age(node) << 11 | node << 16
+    double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef)
+    int *ldet;                // [B*RW] deterministic switch target of block b-1's states
+    double *lprior;           // [RW] first-column prior
+    double *vemit;            // [nvar*RW] emissions of variant columns
     // outputs
-    double *alpha;            // [W] FP64 table
-    float *alpha32;           // [W] FP32 table (normalised columns)
+    void *alpha;              // [L*RW] forward table, double or float
     double *norms;            // [L] column normalisers
     int *path;                // [L]
     double *lnl;              // [1]
 };
 
-// offset of the (c, a, b) entry of the per-block same-branch table
+// state flags in linfo
+enum : int {
+    INFO_TIME = 0xff,
+    INFO_FIRST = 1 << 8,   // first (youngest) state of its branch
+    INFO_LAST = 1 << 9,    // last (oldest) state of its branch
+    INFO_VALID = 1 << 10,  // a real state (not row padding)
+};
+__host__ __device__ inline int info_age(int x) { return (x >> 11) & 31; }
+__host__ __device__ inline int info_node(int x) { return x >> 16; }
+
+// Coefficient planes of a block.  With the forward column x, the
+// within-block recursion at state s = (node n, time b), c = age(n), is
+//   sum_j x_j P(j -> s) = tf[b] + CG_s * P1_s + CK_s * Q_s + CJ_s * x_s
+// where tf[b] = sum_a G[a] P_other(a -> b) over the per-time sums G,
+// P1_s = sum of CF_j x_j over the states j of n younger than s and
+// Q_s = sum of CD_j x_j over the states of n older than s.
+enum : int {
+    CF = 0, CD, CG, CK, CJ,   // same-branch factors
+    CE,                       // emission of an invariant column
+    CRR, CCR,                 // recombination / recoalescence rows of the switch into b
+    CDP,                      // deterministic switch weight of block b-1's states
+    NCOEF
+};
+
+// transposed position of state s in a lane-run row
+__host__ __device__ inline int lane_pos(int s, int R) { return (s % R) * 64 + s / R; }
+
+// offset of the (c, a, b) entry of the per-block same-branch table:
+// sum_{x<c} (T1-x)^2, in closed form
 __host__ __device__ inline int tc_base(int c, int T1)
 {
-    int s = 0;
-    for (int x = 0; x < c; x++) s += (T1 - x) * (T1 - x);
-    return s;
+    return c * T1 * T1 - T1 * c * (c - 1) + (c - 1) * c * (2 * c - 1) / 6;
 }
 
 enum class Precision { FP64 = 0, FP32 = 1 };
@@ -73,8 +95,8 @@ struct RunStats {
     double ms_prep = 0, ms_emit = 0, ms_forward = 0, ms_traceback = 0, ms_total = 0;
     long long W = 0;   // column-states
     int nvar = 0;
-    int fw_passes = 0, tb_passes = 0;   // fix-up passes of the column-tiled schedule
-    long long fw_rerun_cols = 0, tb_rerun_cols = 0;
+    int tb_passes = 0;            // fix-up passes of the tiled traceback
+    long long tb_rerun_cols = 0;  // columns re-sampled by those passes
 };
 
 // One threading instance described on the host.
@@ -84,15 +106,15 @@ struct HostInstance {
     Model model;
 };
 
-// Schedule of a batch run.
+// Schedule of a batch run.  The forward recursion runs one wave per
+// instance over all of its columns (the filter keeps the memory of its start
+// column for ~10^4 columns, so column tiles cannot be started speculatively);
+// the traceback runs one wave per column tile.
 struct BatchOptions {
     Precision prec = Precision::FP64;
-    int chunk_cols = 0;      // FP32 column tile length (0: one tile per instance)
+    int tile_cols = 0;       // traceback tile length (0: one tile per instance)
     bool traceback = true;
-    bool fixup = true;       // false: speculative tiles only (diagnostics)
-    // a re-run FP32 column agrees with the stored one when every entry is
-    // within tol_rel * |stored| + tol_abs (columns sum to one)
-    float tol_rel = 1e-6f, tol_abs = 1e-9f;
+    bool fixup = true;       // false: speculative traceback tiles only (diagnostics)
 };
 
 // A set of independent threading instances resident in HBM.  upload()
@@ -125,7 +147,7 @@ private:
 // Run one instance (a batch of one).  u may be null (no traceback).
 int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha,
                  double *lnl, RunStats *stats, Precision prec = Precision::FP64,
-                 const double *prior = nullptr, int chunk_cols = 0);
+                 const double *prior = nullptr, int tile_cols = 0);
 
 // Number of column-states W and per-block state counts (host, O(B*N)).
 long long instance_states(const FlatTrees &ft, int T, std::vector<int> *S);

@@ -3,39 +3,41 @@
 // Everything runs on one HIP stream over a Batch of independent instances
 // whose inputs (local trees, sequences, uniforms) are resident in HBM.
 //
-//   k_classify_b    one lane per column: masked / invariant / variant
-//   k_scan_b        one workgroup per instance: variant-column numbering
-//   k_prep_b        one workgroup per local-tree block: state space, lineage
-//                   counts, TransMatrix vectors, the (a,b) and (c,a,b)
-//                   transition tables, the switch matrix from the previous
-//                   block, emission terms, first-column prior
-//   k_var_emit_b    one wave per variant column: Felsenstein partials and the
-//                   emission row over the block's states
-//   k_prep32_b      one wave per block: FP32 tables and the time-grouped lane
-//                   assignment used by the FP32 forward
+//   k_classify_b   one lane per column: masked / invariant / variant
+//   k_scan_b       one workgroup per instance: variant-column numbering
+//   k_prep_b       one workgroup per local-tree block: state space, lineage
+//                  counts, TransMatrix vectors and tables, the switch matrix
+//                  from the previous block, emission terms, first-column
+//                  prior, and the per-state coefficient planes of the
+//                  lane-run layout (engine.h)
+//   k_var_emit_b   one wave per variant column: Felsenstein partials and the
+//                  emission row over the block's states
+//   k_fw_lane      one wave per instance over all its columns.  Lane l owns
+//                  states l*R .. l*R+R-1 (node-major).  The same-branch part
+//                  of the transition is a per-branch prefix and suffix sum
+//                  (in-lane scan plus a DPP wave-shift carry chain across
+//                  the few lanes a branch spans), the other-branch part goes
+//                  through the per-time sums (LDS float atomics) and the
+//                  (a,b) table, so a column costs O(S + T^2) with two LDS
+//                  round trips; rows are written coalesced.
+//   k_lnl_b        per-instance sum of log normalisers (FP64)
+//   k_tb_lane      one wave per column tile: FP64 weights and CDF (DPP wave
+//                  scan), rows prefetched two columns ahead.  Tiles after
+//                  the last start from a state sampled from their boundary
+//                  column alone; fix-up passes re-run a tile from the true
+//                  boundary state until the re-sampled path coalesces with
+//                  the stored one (identical from there on).
 //
-// Plain FP64 path (the drop-in, reference arithmetic):
-//   k_forward64_b   one workgroup per instance, previous column in LDS,
-//                   per-time sums by wave reductions, two barriers per column
-//   k_traceback64_b one wave per instance, next row prefetched into
-//                   registers, CDF by a wave scan
-//
-// FP32 column-tiled path (normalised FP32 table, FP64 normalisers and CDF):
-//   k_forward32     one wave per (instance, column tile): each lane owns the
-//                   states of one time bin, so per-time sums are a segmented
-//                   wave scan; one LDS sync per column.  Tiles after the first
-//                   start from a uniform column and are then re-run from the
-//                   true boundary column until they agree with the stored
-//                   speculative columns (fix-up passes).
-//   k_traceback32   one wave per tile, speculative start state; fix-up passes
-//                   re-run from the true boundary state until the re-sampled
-//                   path coalesces with the stored one (exact from there on).
-//   k_lnl_b         per-instance sum of log normalisers
+// The forward kernel is a template over the table type: double is the
+// drop-in arithmetic (the reference's forward table to ~1e-15), float keeps
+// the table and the recursion in FP32 with per-column rescaling while the
+// normalisers, the log-likelihood and the traceback CDF stay FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "engine.h"
@@ -52,10 +54,6 @@ namespace aw {
         }                                                                               \
     } while (0)
 
-// LDS ordering inside a single-wave workgroup: complete this wave's LDS
-// operations and keep the compiler from moving memory accesses across.
-__device__ inline void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
-
 // Pointers read out of DevInst are generic; accesses through them compile to
 // flat instructions, which count against the LDS wait counter too, so every
 // LDS wait would also drain outstanding HBM stores.  gp() restores the
@@ -67,15 +65,20 @@ __device__ inline AW_GLOBAL X *gp(X *p)
     return (AW_GLOBAL X *)p;
 }
 
-// Read-only data indexed by wave-uniform values (block bounds, column
-// classes, uniforms) goes through the constant address space so it is
-// fetched with scalar loads, which do not wait on outstanding vector stores.
+// Read-only data indexed by wave-uniform values (block bounds, uniforms)
+// goes through the constant address space so it is fetched with scalar loads.
 #define AW_CONST __attribute__((address_space(4)))
 template <class X>
 __device__ inline AW_CONST const X *cp(const X *p)
 {
     return (AW_CONST const X *)p;
 }
+
+// LDS operations of one wave are performed in order, so inside a
+// single-wave workgroup only the compiler's order matters; every LDS access
+// of a kernel goes through one dynamic shared array, so accesses that may
+// overlap are never reordered.  This marks the places that rely on it.
+__device__ inline void lds_order() {}
 
 // index i such that off[i] <= x < off[i+1]
 template <class PT, class X>
@@ -88,6 +91,57 @@ __device__ inline int find_owner(PT off, int n, X x)
         else hi = mid - 1;
     }
     return lo;
+}
+
+//===========================================================================
+// cross-lane moves (DPP).  A lane whose source is outside the wave (wave
+// shifts) or outside its 16-lane row (row shifts) receives 0.
+
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_r(float x)
+{
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_r(double x)
+{
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = (unsigned)dpp_i<CTRL>((int)v);
+    const unsigned hi = (unsigned)dpp_i<CTRL>((int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// inclusive prefix sum over the 64 lanes: Hillis-Steele inside each 16-lane
+// row, then the totals of the rows before
+__device__ __forceinline__ double wave_incl_scan(double v)
+{
+    v += dpp_r<DPP_ROW_SHR1>(v);
+    v += dpp_r<DPP_ROW_SHR2>(v);
+    v += dpp_r<DPP_ROW_SHR4>(v);
+    v += dpp_r<DPP_ROW_SHR8>(v);
+    const double t0 = readlane_d(v, 15), t1 = readlane_d(v, 31), t2 = readlane_d(v, 47);
+    const int row = threadIdx.x >> 4;
+    double off = 0.0;
+    if (row >= 1) off = t0;
+    if (row >= 2) off += t1;
+    if (row >= 3) off += t2;
+    return v + off;
 }
 
 //===========================================================================
@@ -176,6 +230,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     const int b = blockIdx.x - blkoff[inst];
     const int tid = threadIdx.x, NT = blockDim.x;
     const int T = d.T, N = d.N, T1 = T - 1;
+    const int R = d.R, RW = 64 * R;
 
     if (tid == 0) sh.M = *d.model;
     for (int i = tid; i < N; i += NT) {
@@ -225,28 +280,13 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     t.root = sh.root;
     pt.root = (b > 0) ? sh.proot : -1;
     const Model &M = sh.M;
+    const TransVec &tv = sh.tv;
     const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-
-    for (int x = tid; x < 9 * T; x += NT) {
-        int v = x / T, a = x % T;
-        double val;
-        switch (v) {
-        case 0: val = sh.tv.D[a]; break;
-        case 1: val = sh.tv.E[a]; break;
-        case 2: val = sh.tv.lnB[a]; break;
-        case 3: val = sh.tv.lnE2[a]; break;
-        case 4: val = sh.tv.lnNegG1[a]; break;
-        case 5: val = sh.tv.G2[a]; break;
-        case 6: val = sh.tv.G3[a]; break;
-        case 7: val = sh.tv.lnG4[a]; break;
-        default: val = sh.tv.norecombs[a]; break;
-        }
-        gp(d.tv)[(size_t)b * 9 * T + x] = val;
-    }
+    auto coef = gp(d.lcoef) + (size_t)b * NCOEF * RW;
+    auto info = gp(d.linfo) + (size_t)b * RW;
+    auto det = gp(d.ldet) + (size_t)b * RW;
 
     for (int n = tid; n < N; n += NT) {
-        gp(d.nfirst)[(size_t)b * N + n] = sh.ni.first[n];
-        gp(d.nhi)[(size_t)b * N + n] = sh.ni.hi[n];
         for (int a = sh.ag[n]; a <= sh.ni.hi[n]; a++) {
             int k = sh.ni.first[n] + a - sh.ag[n];
             gp(d.snode)[so + k] = n;
@@ -264,23 +304,61 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         gp(d.nmut)[2 * ((size_t)b * N + n) + 1] = nomut;
     }
 
-    for (int k = tid; k < S; k += NT) {
-        int n = node_of_state(sh.ni, N, k);
-        int a = sh.ag[n] + k - sh.ni.first[n];
-        EmitTerms e;
-        emit_terms(t, M, sh.mainlen, n, a, e);
-        gp(d.einv)[so + k] = e.inv;
-        for (int x = 0; x < 3; x++) {
-            gp(d.eterm)[6 * (size_t)(so + k) + x] = e.mut[x];
-            gp(d.eterm)[6 * (size_t)(so + k) + 3 + x] = e.nomut[x];
+    // Per-state planes.  For a target state (n, b) with c = age(n), the
+    // reference's same-branch entry tmatrix2[a][k] = get_time(a,b,c,same) -
+    // get_time(a,b,0,other) (sample_thread.cpp:210-226, trans.h:89-128)
+    // factorises as
+    //   a <  b:  D[a] (exp(lnB[a]) - exp(lnNegG1[a]) - B_{c-1}) * E[b] exp(lnE2[b])
+    //   a >  b:  D[a] * E[b] (low_b + G2[b] - ct_b)
+    //   a == b:  D[b] E[b] (low_b + G3[b] - ct_b) + norecombs[b]
+    // with B_{c-1} = exp(lnB[c-1]), low_b = exp(lnE2[b] + lnB[b-1]) and
+    // ct_b = exp(lnG4[b] + lnB[c-1]) (lnG4 = lnE2), so the branch-local
+    // sum is a prefix sum (a < b) and a suffix sum (a > b) along the branch.
+    for (int p = tid; p < RW; p += NT) {
+        const int s = (p & 63) * R + (p >> 6);
+        double cf = 0, cd = 0, cg = 0, ck = 0, cj = 0, ce = 0, pr = 0;
+        int inf = INFO_FIRST | INFO_LAST;
+        if (s < S) {
+            const int n = node_of_state(sh.ni, N, s);
+            const int a = sh.ag[n] + s - sh.ni.first[n];
+            const int c = sh.ag[n], h = sh.ni.hi[n];
+            const double Bc = c > 0 ? exp(tv.lnB[c - 1]) : 0.0;
+            const double low = a > 0 ? exp(tv.lnE2[a] + tv.lnB[a - 1]) : 0.0;
+            const double ct = c > 0 ? exp(tv.lnG4[a] + tv.lnB[c - 1]) : 0.0;
+            cf = tv.D[a] * (exp(tv.lnB[a]) - exp(tv.lnNegG1[a]) - Bc);
+            cd = tv.D[a];
+            cg = tv.E[a] * exp(tv.lnE2[a]);
+            ck = tv.E[a] * (low + tv.G2[a] - ct);
+            cj = tv.D[a] * tv.E[a] * (low + tv.G3[a] - ct) + tv.norecombs[a];
+            EmitTerms e;
+            emit_terms(t, M, sh.mainlen, n, a, e);
+            ce = e.inv;
+            for (int x = 0; x < 3; x++) {
+                gp(d.eterm)[6 * (size_t)(so + s) + x] = e.mut[x];
+                gp(d.eterm)[6 * (size_t)(so + s) + 3 + x] = e.nomut[x];
+            }
+            if (b == 0) pr = prior_time(a, sh.L, M);
+            inf = a | (a == c ? INFO_FIRST : 0) | (a == h ? INFO_LAST : 0) | INFO_VALID |
+                  (c << 11) | (n << 16);
         }
-        if (b == 0) gp(d.prior)[k] = prior_time(a, sh.L, M);
+        coef[CF * RW + p] = cf;
+        coef[CD * RW + p] = cd;
+        coef[CG * RW + p] = cg;
+        coef[CK * RW + p] = ck;
+        coef[CJ * RW + p] = cj;
+        coef[CE * RW + p] = ce;
+        coef[CRR * RW + p] = 0.0;
+        coef[CCR * RW + p] = 0.0;
+        coef[CDP * RW + p] = 0.0;
+        det[p] = -1;
+        info[p] = inf;
+        if (b == 0) gp(d.lprior)[p] = pr;
     }
 
     const int TT = T1 * T1;
     for (int x = tid; x < TT; x += NT) {
         int a = x / T1, bb = x % T1;
-        gp(d.tm)[(size_t)b * TT + x] = sh.tv.get_time(a, bb, 0, false);
+        gp(d.tm)[(size_t)b * TT + x] = tv.get_time(a, bb, 0, false);
     }
     for (int x = tid; x < d.TC; x += NT) {
         int c = 0, base = 0;
@@ -290,38 +368,22 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         }
         int r = x - base, w = T1 - c;
         int a = c + r / w, bb = c + r % w;
-        double same = sh.tv.get_time(a, bb, c, true);
-        gp(d.tm2s)[(size_t)b * d.TC + x] = same;
-        gp(d.tm2)[(size_t)b * d.TC + x] = same - sh.tv.get_time(a, bb, 0, false);
-    }
-
-    // states grouped by time (counting sort, stable in state order)
-    if (tid == 0) {
-        int cnt[MAXT + 1];
-        for (int a = 0; a <= T; a++) cnt[a] = 0;
-        for (int n = 0; n < N; n++)
-            for (int a = sh.ag[n]; a <= sh.ni.hi[n]; a++) cnt[a + 1]++;
-        for (int a = 0; a < T; a++) cnt[a + 1] += cnt[a];
-        for (int a = 0; a <= T; a++) gp(d.tstart)[(size_t)b * (T + 1) + a] = cnt[a];
-        for (int n = 0; n < N; n++)
-            for (int a = sh.ag[n]; a <= sh.ni.hi[n]; a++)
-                gp(d.tperm)[so + cnt[a]++] = sh.ni.first[n] + a - sh.ag[n];
+        gp(d.tm2s)[(size_t)b * d.TC + x] = tv.get_time(a, bb, c, true);
     }
 
     if (b == 0) return;
+    __syncthreads();   // planes cleared before the switch entries are set
 
-    // switch matrix from block b-1 (calc_transition_probs_switch)
+    // switch matrix from block b-1 (calc_transition_probs_switch), entries
+    // of the previous block's states at their lane-run positions
     const int so1 = cp(d.soff)[b - 1], S1 = so - so1;
     for (int j = tid; j < S1; j += NT) {
         int n1 = node_of_state(sh.pni, N, j);
         int t1 = sh.pag[n1] + j - sh.pni.first[n1];
         int tgt = sw_determ(pt, t, sh.ni, spr, sh.map, n1, t1);
-        gp(d.determ)[so1 + j] = tgt;
-        gp(d.determp)[so1 + j] = tgt >= 0 ? sw_determprob(pt, M, sh.PL, spr, sh.cx, n1, t1) : 0.0;
-    }
-    for (int k = tid; k < S; k += NT) {
-        gp(d.recombrow)[so + k] = 0.0;
-        gp(d.recoalrow)[so + k] = 0.0;
+        const int p = lane_pos(j, R);
+        det[p] = tgt;
+        coef[CDP * RW + p] = tgt >= 0 ? sw_determprob(pt, M, sh.PL, spr, sh.cx, n1, t1) : 0.0;
     }
     if (tid == 0) {
         int rs = lookup(pt, sh.pni, spr.rn, spr.rt);
@@ -352,11 +414,11 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         int stay = lookup(t, sh.ni, sh.map[spr.rn], spr.rt);
         int esc = lookup(t, sh.ni, node2, time2);
         if (stay != -1)
-            gp(d.recombrow)[so + stay] = sw_recomb_recoal(pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt,
-                                                      sh.cx.rpa, sh.cx.last_len);
+            coef[CRR * RW + lane_pos(stay, R)] = sw_recomb_recoal(
+                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, sh.cx.rpa, sh.cx.last_len);
         if (esc != -1)
-            gp(d.recombrow)[so + esc] = sw_recomb_recoal(pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt,
-                                                     spr.rt, sh.cx.last_len);
+            coef[CRR * RW + lane_pos(esc, R)] = sw_recomb_recoal(
+                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, spr.rt, sh.cx.last_len);
     }
     if (sh.recoalsrc != -1) {
         const int time1 = spr.ct;
@@ -367,31 +429,14 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             if (!((n2 == mrn && t2 >= spr.rt) || (n2 == sh.node3 && t2 == time1) ||
                   (n2 == sh.rparent && t2 == time1)))
                 continue;
-            gp(d.recoalrow)[so + k] = sw_recomb_recoal(pt, M, sh.PL, spr, t2, spr.cn, spr.ct,
-                                                   sh.cx.rpa, sh.cx.last_len);
+            coef[CCR * RW + lane_pos(k, R)] = sw_recomb_recoal(pt, M, sh.PL, spr, t2, spr.cn,
+                                                               spr.ct, sh.cx.rpa, sh.cx.last_len);
         }
-    }
-    __syncthreads();
-    // inverse of the deterministic map, sources ascending per target
-    if (tid == 0) {
-        auto ptr = gp(d.invptr) + so + b;
-        for (int k = 0; k <= S; k++) ptr[k] = 0;
-        for (int j = 0; j < S1; j++) {
-            int k = gp(d.determ)[so1 + j];
-            if (j != sh.recombsrc && j != sh.recoalsrc && k != -1) ptr[k + 1]++;
-        }
-        for (int k = 0; k < S; k++) ptr[k + 1] += ptr[k];
-        for (int j = 0; j < S1; j++) {
-            int k = gp(d.determ)[so1 + j];
-            if (j != sh.recombsrc && j != sh.recoalsrc && k != -1) gp(d.invidx)[so1 + ptr[k]++] = j;
-        }
-        for (int k = S; k > 0; k--) ptr[k] = ptr[k - 1];
-        ptr[0] = 0;
     }
 }
 
 //===========================================================================
-// emissions of variant columns
+// emissions of variant columns, written as lane-run rows
 
 __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D,
                                                    const int *__restrict__ voff, int ninst)
@@ -405,7 +450,7 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
     const DevInst &d = D[inst];
     const int v = blockIdx.x - voff[inst];
     const int lane = threadIdx.x;
-    const int N = d.N;
+    const int N = d.N, R = d.R, RW = 64 * R;
     const int col = gp(d.vcol)[v];
     const int b = find_owner(cp(d.bstart), d.B, col);
     const int nleaves = (N + 1) / 2;
@@ -429,753 +474,314 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
     }
     __syncthreads();
     const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-    auto row = gp(d.vemit) + (size_t)v * d.Smax;
-    for (int k = lane; k < S; k += 64) {
-        int n = gp(d.snode)[so + k];
-        const double *et = d.eterm + 6 * (size_t)(so + k);
-        row[k] = emit_variant(in2, inner + 4 * n, outer + 4 * n, n == root, et, et + 3);
+    auto row = gp(d.vemit) + (size_t)v * RW;
+    for (int p = lane; p < RW; p += 64) {
+        const int k = (p & 63) * R + (p >> 6);
+        double e = 0.0;
+        if (k < S) {
+            int n = gp(d.snode)[so + k];
+            const double *et = d.eterm + 6 * (size_t)(so + k);
+            e = emit_variant(in2, inner + 4 * n, outer + 4 * n, n == root, et, et + 3);
+        }
+        row[p] = e;
     }
 }
 
 //===========================================================================
-// FP32 tables and lane assignment: lanes are grouped by time bin so that a
-// lane's slots all share one time; R slots per lane is the smallest value
-// for which the bins fit in 64 lanes.
+// forward recursion, one wave per instance (arghmm_forward_alg,
+// sample_thread.cpp:394-460; arghmm_forward_block :186-296;
+// arghmm_forward_switch :345-389)
 
-__global__ __launch_bounds__(64) void k_prep32_b(const DevInst *__restrict__ D,
-                                                 const int *__restrict__ blkoff, int ninst,
-                                                 int *overflow)
+// NT1 is a compile-time bound on the T-1 time bins (19 for the usual 20
+// time points), so the per-time loops are fully unrolled without padding
+template <class Real, int R, int NT1>
+struct FwLane {
+    Real cf[R], cd[R], cg[R], ck[R], cj[R], ce[R];
+    int ti[R];
+    bool first_in_lane, last_in_lane;
+    Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table
+};
+
+template <class Real, int R, int NT1>
+__device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Real, R, NT1> &fl)
 {
-    const int inst = find_owner(blkoff, ninst, (int)blockIdx.x);
-    const DevInst &d = D[inst];
-    const int b = blockIdx.x - blkoff[inst];
+    constexpr int RW = 64 * R;
+    const int lane = threadIdx.x, T1 = d.T - 1;
+    // uniform bases, per-lane index: loads use the scalar-base address form
+    const AW_GLOBAL double *coef = gp(d.lcoef) + (size_t)b * NCOEF * RW;
+    const AW_GLOBAL int *info = gp(d.linfo) + (size_t)b * RW;
+    fl.first_in_lane = false;
+    fl.last_in_lane = false;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int x = r * 64 + lane;
+        fl.cf[r] = (Real)coef[CF * RW + x];
+        fl.cd[r] = (Real)coef[CD * RW + x];
+        fl.cg[r] = (Real)coef[CG * RW + x];
+        fl.ck[r] = (Real)coef[CK * RW + x];
+        fl.cj[r] = (Real)coef[CJ * RW + x];
+        fl.ce[r] = (Real)coef[CE * RW + x];
+        fl.ti[r] = info[x];
+        fl.first_in_lane |= (fl.ti[r] & INFO_FIRST) != 0;
+        fl.last_in_lane |= (fl.ti[r] & INFO_LAST) != 0;
+    }
+    // Loaded values go straight into their registers: selecting or copying
+    // a value whose load is still in flight makes the compiler wait for it
+    // at once.  Indices are clamped instead; entries a >= T-1 meet a zero
+    // per-time sum and lanes >= T-1 write TF entries nobody reads.
+    const AW_GLOBAL double *tmb = gp(d.tm) + (size_t)b * T1 * T1;
+    const int lc = min(lane, T1 - 1);
+#pragma unroll
+    for (int a = 0; a < NT1; a++) fl.tmc[a] = (Real)tmb[min(a, T1 - 1) * T1 + lc];
+}
+
+// variant row v, clamped to the last row (an instance without variant
+// columns has one allocated row that is never used), kept as loaded until
+// its column needs it
+template <int R>
+__device__ __forceinline__ void vrow_load(const DevInst &d, int v, double (&ev)[R])
+{
+    constexpr int RW = 64 * R;
+    const AW_GLOBAL double *row = gp(d.vemit) + (size_t)max(min(v, d.nvar - 1), 0) * RW;
+#pragma unroll
+    for (int r = 0; r < R; r++) ev[r] = row[r * 64 + threadIdx.x];
+}
+
+template <class Real>
+__device__ __forceinline__ void lds_add(Real *p, Real v)
+{
+    (void)atomicAdd(p, v);
+}
+
+// d is taken by value: its fields stay in scalar registers for the whole
+// loop instead of being re-read from the instance array
+// variant rows staged per LDS refill (at most 16 KB of them)
+template <class Real, int R>
+constexpr int fw_vst()
+{
+    return (16384 / (64 * R * (int)sizeof(Real))) < 8 ? (16384 / (64 * R * (int)sizeof(Real))) : 8;
+}
+
+template <class Real, int R, int NT1>
+__device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
+{
+    constexpr int RW = 64 * R;
+    constexpr int VST = fw_vst<Real, R>();
     const int lane = threadIdx.x;
-    const int T = d.T, T1 = T - 1, TT = T1 * T1, RM = d.RM;
-    const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-    for (int x = lane; x < TT; x += 64) gp(d.tm32)[(size_t)b * TT + x] = (float)gp(d.tm)[(size_t)b * TT + x];
-    for (int x = lane; x < d.TC; x += 64) gp(d.tm2_32)[(size_t)b * d.TC + x] = (float)gp(d.tm2)[(size_t)b * d.TC + x];
-    for (int k = lane; k < S; k += 64) gp(d.einv32)[so + k] = (float)gp(d.einv)[so + k];
-    for (int x = lane; x < 64 * RM; x += 64) gp(d.lslot)[(size_t)b * 64 * RM + x] = -1;
-    gp(d.lkey)[(size_t)b * 64 + lane] = T1;
-    __syncthreads();
-    if (lane == 0) {
-        const int *ts = d.tstart + (size_t)b * (T + 1);
-        int R = 1;
-        for (; R <= RM; R++) {
-            int lanes = 0;
-            for (int a = 0; a < T1; a++) lanes += (ts[a + 1] - ts[a] + R - 1) / R;
-            if (lanes <= 64) break;
+    const int T1 = d.T - 1, L = d.L;
+    // carry steps: a branch has at most T-1 states, so it reaches back at
+    // most ceil((T-2)/R) lanes from the lane holding its last state
+    const int K = (T1 - 1 + R - 1) / R;
+    Real *G2buf = sm;                // two 32-entry per-time sum buffers
+    Real *TF = sm + 64;              // 32: other-branch term per target time
+    Real *XS = sm + 96;              // RW: previous column in state order (switch)
+    Real *NW = XS + RW;              // RW: switch targets
+    Real *VS = NW + RW;              // VST rows: staged variant-column emissions
+    AW_GLOBAL Real *alpha = (AW_GLOBAL Real *)d.alpha;
+    auto norms = gp(d.norms);
+
+    FwLane<Real, R, NT1> fl;
+    fw_load_block<Real, R, NT1>(d, 0, fl);
+    int b = 0, bend = cp(d.blen)[0];
+    if (lane < 32) {
+        G2buf[lane] = Real(0);
+        G2buf[32 + lane] = Real(0);
+    }
+    // Column classes come through scalar loads one column ahead: the LDS
+    // waits of a column (lgkmcnt) cover them, while a vector load would
+    // order behind the column's HBM stores (vmcnt counts both).
+    auto clsp = cp(d.cls);
+    int cls_next = clsp[0];
+    // Variant rows are consumed in order and staged in LDS, VST at a time:
+    // a refill is one batch of loads and one wait, and the columns read
+    // their row from LDS.  (vmcnt counts loads and stores in order, so any
+    // per-column HBM load would also wait for the previous column's stores.)
+    int vi = 0, vbase = 0, vend = 0;
+
+    Real x[R], tfs[R], v[R];
+    Real sc = Real(1);   // the previous column is x * sc
+
+    for (int col = 0; col < L; col++) {
+        const int cls = cls_next;
+        cls_next = clsp[min(col + 1, L - 1)];
+        Real *Gc = G2buf + ((col & 1) << 5);
+        Real *Gz = G2buf + (((col + 1) & 1) << 5);
+        // the variant row (column 0 takes no emission but still owns its row)
+        Real ev[R];
+        const bool var = cls >= 0;
+        if (var) {
+            if (vi >= vend) {
+                vbase = vi;
+                vend = min(vi + VST, d.nvar);
+                double buf[VST][R];
+#pragma unroll
+                for (int k = 0; k < VST; k++) vrow_load<R>(d, vbase + k, buf[k]);
+#pragma unroll
+                for (int k = 0; k < VST; k++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) VS[k * RW + r * 64 + lane] = (Real)buf[k][r];
+                lds_order();
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) ev[r] = VS[(vi - vbase) * RW + r * 64 + lane];
+            vi++;
         }
-        if (R > RM) {
-            atomicMax(overflow, R);
-            return;
-        }
-        int l = 0;
-        for (int a = 0; a < T1; a++) {
-            for (int x = ts[a]; x < ts[a + 1]; x += R) {
-                int e = min(ts[a + 1], x + R);
-                for (int y = x; y < e; y++)
-                    gp(d.lslot)[((size_t)b * 64 + l) * RM + (y - x)] = gp(d.tperm)[so + y];
-                gp(d.lkey)[(size_t)b * 64 + l] = a;
-                l++;
+
+        if (col == 0) {
+            // unnormalised prior, not multiplied by the emission
+            // (calc_state_priors, trans.cpp:785-831)
+            // padding positions of the prior row hold 0
+            const AW_GLOBAL double *pr = gp(d.lprior);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = (Real)pr[r * 64 + lane];
+        } else {
+            const bool sw = (col == bend);
+            Real rr[R], cr[R];
+            Real prs = Real(0), pcs = Real(0);
+            int rs = -1, cs = -1, S2 = 0;
+            if (sw) {
+                // deterministic switch transitions scattered into NW; the
+                // recombination and recoalescence sources are excluded
+                const int nb = b + 1;
+                const int so0 = cp(d.soff)[b], so1 = cp(d.soff)[nb];
+                const int S1 = so1 - so0;
+                S2 = cp(d.soff)[nb + 1] - so1;
+                rs = cp(d.srcs)[2 * nb];
+                cs = cp(d.srcs)[2 * nb + 1];
+                const AW_GLOBAL double *coefn = gp(d.lcoef) + (size_t)nb * NCOEF * RW;
+                const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)nb * RW;
+                int dt[R];
+                Real dpv[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int x = r * 64 + lane;
+                    dt[r] = detn[x];
+                    dpv[r] = (Real)coefn[CDP * RW + x];
+                    rr[r] = (Real)coefn[CRR * RW + x];
+                    cr[r] = (Real)coefn[CCR * RW + x];
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int s = lane * R + r;
+                    if (s < S1) XS[s] = x[r];
+                }
+                for (int k = lane; k < S2; k += 64) NW[k] = Real(0);
+                lds_order();
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int s = lane * R + r;
+                    if (s < S1 && dt[r] >= 0 && s != rs && s != cs) lds_add(&NW[dt[r]], x[r] * dpv[r]);
+                }
+                lds_order();
+                if (rs != -1) prs = XS[rs] * sc;
+                if (cs != -1) pcs = XS[cs] * sc;
+                b = nb;
+                bend += cp(d.blen)[nb];
+                fw_load_block<Real, R, NT1>(d, b, fl);
+            }
+            // emission of the column
+            Real e[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) e[r] = var ? ev[r] : (cls == -1) ? fl.ce[r] : Real(1);
+            if (sw) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int s = lane * R + r;
+                    const Real nw = NW[min(s, RW - 1)];
+                    Real acc = (s < S2) ? nw * sc : Real(0);
+                    if (rs != -1 && rr[r] > Real(0)) acc += prs * rr[r];
+                    if (cs != -1 && cr[r] > Real(0)) acc += pcs * cr[r];
+                    v[r] = (fl.ti[r] & INFO_VALID) ? acc * e[r] : Real(0);
+                }
+            } else {
+                // same-branch prefix sums (younger states) with the carry
+                // from the lanes before, suffix sums (older states) with the
+                // carry from the lanes after.  Branch ends are applied as 0/1
+                // factors converted from the info word: compare-and-select
+                // forms keep one exec mask per slot alive in scalar registers
+                Real P[R], Q[R];
+                Real p = Real(0);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const Real nf = (Real)(1 - ((fl.ti[r] >> 8) & 1));   // not INFO_FIRST
+                    P[r] = p * nf;
+                    p = P[r] + fl.cf[r] * x[r];
+                }
+                Real q = Real(0);
+#pragma unroll
+                for (int r = R - 1; r >= 0; r--) {
+                    const Real nl = (Real)(1 - ((fl.ti[r] >> 9) & 1));   // not INFO_LAST
+                    Q[r] = q * nl;
+                    q = Q[r] + fl.cd[r] * x[r];
+                }
+                const Real m1 = fl.first_in_lane ? Real(0) : Real(1);
+                const Real m2 = fl.last_in_lane ? Real(0) : Real(1);
+                Real c1 = Real(0), c2 = Real(0);
+                for (int k = 0; k < K; k++) {
+                    c1 = dpp_r<DPP_WAVE_SHR1>(p + m1 * c1);
+                    c2 = dpp_r<DPP_WAVE_SHL1>(q + m2 * c2);
+                }
+                // the carry reaches the slots before the lane's first branch start
+                // (prefix) / after its last branch end (suffix)
+                Real f1 = Real(1);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    f1 *= (Real)(1 - ((fl.ti[r] >> 8) & 1));
+                    P[r] += c1 * f1;
+                }
+                Real f2 = Real(1);
+#pragma unroll
+                for (int r = R - 1; r >= 0; r--) {
+                    f2 *= (Real)(1 - ((fl.ti[r] >> 9) & 1));
+                    Q[r] += c2 * f2;
+                }
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const Real vm = (Real)((fl.ti[r] >> 10) & 1);   // INFO_VALID
+                    const Real same = (fl.cg[r] * P[r] + fl.ck[r] * Q[r]) + fl.cj[r] * x[r];
+                    v[r] = vm * (e[r] * (tfs[r] + sc * same));
+                }
             }
         }
-    }
-}
 
-//===========================================================================
-// plain FP64 forward, one workgroup per instance
-
-__device__ inline double wave_sum(double x)
-{
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-template <int NT>
-__device__ inline void time_sums(const double *col, const int *tperm, const int *tstart, int T1,
-                                 double *G)
-{
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int a = w; a < T1; a += NW) {
-        double s = 0.0;
-        for (int x = tstart[a] + lane; x < tstart[a + 1]; x += 64) s += col[tperm[x]];
-        s = wave_sum(s);
-        if (lane == 0) G[a] = s;
-    }
-}
-
-template <int NT, int R>
-__global__ __launch_bounds__(NT) void k_forward64_b(const DevInst *__restrict__ D,
-                                                    const int *__restrict__ list)
-{
-    extern __shared__ double dyn[];
-    const DevInst &d = D[list[blockIdx.x]];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int T1 = d.T - 1, TT = T1 * T1, N = d.N;
-    double *bufA = dyn;
-    double *bufB = bufA + d.Smax;
-    double *tm2 = bufB + d.Smax;
-    double *G = tm2 + d.TC;                      // 32 doubles
-    int *tperm = (int *)(G + 32);                // Smax ints
-    int *tstart = tperm + d.Smax;                // 33 ints
-
-    double *cur = bufA, *nxt = bufB;
-    double sc = 1.0;        // alpha of the column held in cur = cur * sc
-    int prevS = 0;
-    long long prev_row = 0;
-    double tmc[MAXT - 1];   // lane b < T1 keeps column b of the (a,b) table
-
-    for (int b = 0; b < d.B; b++) {
-        const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-        const int start = cp(d.bstart)[b], len = cp(d.blen)[b];
-        const long long aoff = cp(d.aoff)[b];
-
-        for (int x = tid; x < d.TC; x += NT) tm2[x] = gp(d.tm2)[(size_t)b * d.TC + x];
-        for (int x = tid; x < S; x += NT) tperm[x] = gp(d.tperm)[so + x];
-        for (int x = tid; x <= d.T; x += NT) tstart[x] = gp(d.tstart)[(size_t)b * (d.T + 1) + x];
+        // per-time sums of the new column, its normaliser, the stored row
+        // and the other-branch term of the next column (padding slots add 0)
 #pragma unroll
-        for (int a = 0; a < MAXT - 1; a++)
-            tmc[a] = (a < T1 && lane < T1) ? gp(d.tm)[(size_t)b * TT + a * T1 + lane] : 0.0;
-
-        int sb[R], sj1[R], slen[R], sbase[R], sstride[R];
-        double se[R];
+        for (int r = 0; r < R; r++) lds_add(&Gc[fl.ti[r] & INFO_TIME], v[r]);
+        lds_order();
+        Real norm = Real(0), t = Real(0);
+#pragma unroll
+        for (int a = 0; a < NT1; a++) {
+            const Real g = Gc[a];
+            norm += g;
+            t += g * fl.tmc[a];
+        }
+        // the first column is stored unnormalised (scale 1)
+        const Real nsc = (col == 0) ? Real(1) : Real(1) / norm;
+        AW_GLOBAL Real *row = alpha + (size_t)col * RW;
+#pragma unroll
+        for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
+        if (lane == 0) norms[col] = (col == 0) ? ((L == 1) ? (double)norm : 1.0) : (double)norm;
+        if (lane < 32) {
+            TF[lane] = t * nsc;
+            Gz[lane] = Real(0);
+        }
+        lds_order();
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            int k = tid + NT * r;
-            sb[r] = 0; sj1[r] = 0; slen[r] = 0; sbase[r] = 0; sstride[r] = 1; se[r] = 0.0;
-            if (k < S) {
-                int n = gp(d.snode)[so + k];
-                int bt = gp(d.stime)[so + k];
-                int lo = gp(d.age)[(size_t)b * N + n];
-                int hi = gp(d.nhi)[(size_t)b * N + n];
-                sb[r] = bt;
-                sj1[r] = gp(d.nfirst)[(size_t)b * N + n];
-                slen[r] = hi - lo + 1;
-                sstride[r] = T1 - lo;
-                sbase[r] = tc_base(lo, T1) + (bt - lo);
-                se[r] = gp(d.einv)[so + k];
-            }
+            tfs[r] = TF[fl.ti[r] & INFO_TIME];
+            x[r] = v[r];
         }
-        int cls_next = (len > 1) ? cp(d.cls)[start + 1] : 0;
-
-        // column `start`: prior (block 0) or switch from the previous block
-        {
-            const int cls = cp(d.cls)[start];
-            if (b == 0) {
-                for (int k = tid; k < S; k += NT) nxt[k] = gp(d.prior)[k];
-            } else {
-                const int rs = cp(d.srcs)[2 * b], cs = cp(d.srcs)[2 * b + 1];
-                const int so1 = cp(d.soff)[b - 1];
-                auto ptr = gp(d.invptr) + so + b;
-                const double prs = (rs != -1) ? cur[rs] * sc : 0.0;
-                const double pcs = (cs != -1) ? cur[cs] * sc : 0.0;
-                for (int k = tid; k < S; k += NT) {
-                    double acc = 0.0;
-                    for (int x = ptr[k]; x < ptr[k + 1]; x++) {
-                        int j = gp(d.invidx)[so1 + x];
-                        acc += (cur[j] * sc) * gp(d.determp)[so1 + j];
-                    }
-                    double rr = gp(d.recombrow)[so + k], cr = gp(d.recoalrow)[so + k];
-                    if (rs != -1 && rr > 0.0) acc += prs * rr;
-                    if (cs != -1 && cr > 0.0) acc += pcs * cr;
-                    double e = (cls == -1) ? gp(d.einv)[so + k]
-                             : (cls == -2) ? 1.0 : gp(d.vemit)[(size_t)cls * d.Smax + k];
-                    nxt[k] = acc * e;
-                }
-            }
-            __syncthreads();
-            if (b > 0)
-                for (int k = tid; k < prevS; k += NT) gp(d.alpha)[prev_row + k] = cur[k] * sc;
-            time_sums<NT>(nxt, tperm, tstart, T1, G);
-            __syncthreads();
-            double norm = 0.0;
-            for (int a = 0; a < T1; a++) norm += G[a];
-            if (b == 0) {
-                sc = 1.0;   // the prior column is kept unnormalised
-                if (tid == 0) gp(d.norms)[0] = (d.L == 1) ? norm : 1.0;
-            } else {
-                sc = 1.0 / norm;
-                if (tid == 0) gp(d.norms)[start] = norm;
-            }
-            double *tmp = cur; cur = nxt; nxt = tmp;
-        }
-
-        for (int i = 1; i < len; i++) {
-            const int col = start + i;
-            const int cls = cls_next;
-            if (i + 1 < len) cls_next = cp(d.cls)[col + 1];
-            const long long row = aoff + (long long)(i - 1) * S;
-            double tfb = 0.0;
-#pragma unroll
-            for (int a = 0; a < MAXT - 1; a++)
-                if (a < T1) tfb += tmc[a] * G[a];
-            tfb *= sc;
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int k = tid + NT * r;
-                double tfv = __shfl(tfb, sb[r], 64);
-                if (k < S) {
-                    gp(d.alpha)[row + k] = cur[k] * sc;
-                    double s = 0.0;
-                    const double *coef = tm2 + sbase[r];
-                    const double *src = cur + sj1[r];
-                    const int st = sstride[r];
-                    for (int x = 0; x < slen[r]; x++) s += coef[x * st] * src[x];
-                    double e = (cls == -1) ? se[r]
-                             : (cls == -2) ? 1.0 : gp(d.vemit)[(size_t)cls * d.Smax + k];
-                    nxt[k] = (tfv + sc * s) * e;
-                }
-            }
-            __syncthreads();
-            time_sums<NT>(nxt, tperm, tstart, T1, G);
-            __syncthreads();
-            double norm = 0.0;
-            for (int a = 0; a < T1; a++) norm += G[a];
-            sc = 1.0 / norm;
-            if (tid == 0) gp(d.norms)[col] = norm;
-            double *tmp = cur; cur = nxt; nxt = tmp;
-        }
-        prevS = S;
-        prev_row = aoff + (long long)(len - 1) * S;
-        __syncthreads();
-    }
-    for (int k = tid; k < prevS; k += NT) gp(d.alpha)[prev_row + k] = cur[k] * sc;
-}
-
-//===========================================================================
-// traceback sampling helper
-
-// first index j of w[0..n) with cumulative sum >= u * total (sample(),
-// common.h:272-290); each lane owns a contiguous chunk of the weights
-__device__ inline int wave_sample(const double *w, int n, double u)
-{
-    const int lane = threadIdx.x & 63;
-    const int chunk = (n + 63) / 64;
-    const int s0 = min(n, lane * chunk), e0 = min(n, s0 + chunk);
-    double loc = 0.0;
-    for (int j = s0; j < e0; j++) loc += w[j];
-    double incl = loc;
-    for (int off = 1; off < 64; off <<= 1) {
-        double y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    double excl = __shfl_up(incl, 1, 64);
-    if (lane == 0) excl = 0.0;
-    double mine = excl + loc;
-    double total = __shfl(mine, 63, 64);
-    double pick = u * total;
-    unsigned long long m = __ballot(mine >= pick && e0 > s0);
-    if (m == 0) return n - 1;
-    int owner = __ffsll((long long)m) - 1;
-    int res = n - 1;
-    if (lane == owner) {
-        double p = 0.0;
-        res = e0 - 1;
-        for (int j = s0; j < e0; j++) {
-            p += w[j];
-            if (excl + p >= pick) { res = j; break; }
-        }
-    }
-    return __shfl(res, owner, 64);
-}
-
-// column table layout shared by both tracebacks
-struct TbTables {
-    double *w, *tm, *tm2s;
-    int *snode, *stime;
-};
-
-__device__ inline void tb_load_tables(const DevInst &d, int b, const TbTables &tt)
-{
-    const int lane = threadIdx.x;
-    const int T1 = d.T - 1, TT = T1 * T1;
-    const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-    for (int x = lane; x < TT; x += 64) tt.tm[x] = gp(d.tm)[(size_t)b * TT + x];
-    for (int x = lane; x < d.TC; x += 64) tt.tm2s[x] = gp(d.tm2s)[(size_t)b * d.TC + x];
-    for (int x = lane; x < S; x += 64) {
-        tt.snode[x] = gp(d.snode)[so + x];
-        tt.stime[x] = gp(d.stime)[so + x];
-    }
-}
-
-// weight of the within-block transition j -> knext
-__device__ inline double tb_trans(const DevInst &d, const TbTables &tt, int b, int j, int nk,
-                                  int bk, int ck)
-{
-    const int T1 = d.T - 1;
-    int a = tt.stime[j];
-    if (tt.snode[j] == nk) return tt.tm2s[tc_base(ck, T1) + (a - ck) * (T1 - ck) + (bk - ck)];
-    return tt.tm[a * T1 + bk];
-}
-
-// weight of the switch transition j (block b-1) -> knext (block b)
-__device__ inline double tb_switch(const DevInst &d, int b, int j, int knext)
-{
-    const int so = cp(d.soff)[b], so1 = cp(d.soff)[b - 1];
-    const int rs = cp(d.srcs)[2 * b], cs = cp(d.srcs)[2 * b + 1];
-    if (j == cs) return gp(d.recoalrow)[so + knext];
-    if (j == rs) return gp(d.recombrow)[so + knext];
-    return (gp(d.determ)[so1 + j] == knext) ? gp(d.determp)[so1 + j] : 0.0;
-}
-
-//===========================================================================
-// traceback over a column range, one wave; the next row is prefetched into
-// registers while the current one is sampled
-
-template <class PT, class X, int RT>
-__device__ inline void load_row(PT row, int S, X (&reg)[RT])
-{
-    const int lane = threadIdx.x;
-#pragma unroll
-    for (int r = 0; r < RT; r++) {
-        int j = lane + 64 * r;
-        reg[r] = (j < S) ? row[j] : X(0);
-    }
-}
-
-// Sample path[col] for col = hi down to lo given the state knext at column
-// hi+1 (in block kblk).  mode 1 stops at the first column whose sample
-// equals the stored path (the chains have coalesced) and returns it; else
-// returns -1.  Uniform of column col: u[L-1-col].
-template <class AT, int RT>
-__device__ int tb_range(const DevInst &d, AW_GLOBAL const AT *alpha, const TbTables &tt,
-                        int hi, int lo, int knext, int kblk, int mode)
-{
-    const int lane = threadIdx.x;
-    const int L = d.L, N = d.N;
-    auto u = cp(d.u);
-    auto path = gp(d.path);
-    int bc = find_owner(cp(d.bstart), d.B, hi);
-    int bc_start = cp(d.bstart)[bc], bc_S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
-    long long bc_aoff = cp(d.aoff)[bc];
-    // the block before, fetched ahead of need
-    int pb_start = 0, pb_S = 0;
-    long long pb_aoff = 0;
-    if (bc > 0) {
-        pb_start = cp(d.bstart)[bc - 1];
-        pb_S = cp(d.soff)[bc] - cp(d.soff)[bc - 1];
-        pb_aoff = cp(d.aoff)[bc - 1];
-    }
-    int tblk = -1;
-    AT pf[RT], pn[RT];
-    if (hi >= lo) load_row(alpha + bc_aoff + (long long)(hi - bc_start) * bc_S, bc_S, pn);
-    int result = -1;
-    for (int col = hi; col >= lo; col--) {
-#pragma unroll
-        for (int r = 0; r < RT; r++) pf[r] = pn[r];
-        const int oldp = (mode == 1) ? path[col] : -1;
-        if (col - 1 >= lo) {
-            if (col - 1 >= bc_start)
-                load_row(alpha + bc_aoff + (long long)(col - 1 - bc_start) * bc_S, bc_S, pn);
-            else
-                load_row(alpha + pb_aoff + (long long)(col - 1 - pb_start) * pb_S, pb_S, pn);
-        }
-        if (bc != kblk) {
-            // col is the last column of block bc = kblk-1: switch into kblk
-#pragma unroll
-            for (int r = 0; r < RT; r++) {
-                int j = lane + 64 * r;
-                if (j < bc_S) tt.w[j] = (double)pf[r] * tb_switch(d, kblk, j, knext);
-            }
-        } else {
-            if (tblk != bc) {
-                tb_load_tables(d, bc, tt);
-                tblk = bc;
-                __syncthreads();
-            }
-            const int nk = tt.snode[knext], bk = tt.stime[knext];
-            const int ck = gp(d.age)[(size_t)bc * N + nk];
-#pragma unroll
-            for (int r = 0; r < RT; r++) {
-                int j = lane + 64 * r;
-                if (j < bc_S) tt.w[j] = (double)pf[r] * tb_trans(d, tt, bc, j, nk, bk, ck);
-            }
-        }
-        __syncthreads();
-        knext = wave_sample(tt.w, bc_S, u[L - 1 - col]);
-        kblk = bc;
-        if (mode == 1 && oldp == knext) {
-            result = col;
-            break;
-        }
-        if (lane == 0) path[col] = knext;
-        __syncthreads();
-        if (col - 1 < bc_start && col - 1 >= lo) {
-            bc--;
-            bc_start = pb_start;
-            bc_S = pb_S;
-            bc_aoff = pb_aoff;
-            if (bc > 0) {
-                pb_start = cp(d.bstart)[bc - 1];
-                pb_S = cp(d.soff)[bc] - cp(d.soff)[bc - 1];
-                pb_aoff = cp(d.aoff)[bc - 1];
-            }
-        }
-    }
-    return result;
-}
-
-// sample the last column of the instance from its forward column alone
-template <class AT, int RT>
-__device__ int tb_sample_column(const DevInst &d, AW_GLOBAL const AT *alpha,
-                                const TbTables &tt, int col, double uval, int *blk)
-{
-    const int lane = threadIdx.x;
-    const int b = find_owner(cp(d.bstart), d.B, col);
-    const int S = cp(d.soff)[b + 1] - cp(d.soff)[b];
-    AT pf[RT];
-    load_row(alpha + cp(d.aoff)[b] + (long long)(col - cp(d.bstart)[b]) * S, S, pf);
-#pragma unroll
-    for (int r = 0; r < RT; r++)
-        if (lane + 64 * r < S) tt.w[lane + 64 * r] = (double)pf[r];
-    __syncthreads();
-    int k = wave_sample(tt.w, S, uval);
-    __syncthreads();
-    *blk = b;
-    return k;
-}
-
-__device__ inline TbTables tb_layout(double *dyn, const DevInst &d)
-{
-    const int T1 = d.T - 1;
-    TbTables tt;
-    tt.w = dyn;
-    tt.tm = tt.w + d.Smax;
-    tt.tm2s = tt.tm + T1 * T1;
-    tt.snode = (int *)(tt.tm2s + d.TC);
-    tt.stime = tt.snode + d.Smax;
-    return tt;
-}
-
-// plain FP64 traceback of whole instances
-template <int RT>
-__global__ __launch_bounds__(64) void k_traceback64_b(const DevInst *__restrict__ D,
-                                                      const int *__restrict__ list)
-{
-    extern __shared__ double dyn[];
-    const DevInst &d = D[list[blockIdx.x]];
-    TbTables tt = tb_layout(dyn, d);
-    const int L = d.L;
-    int kblk;
-    int k = tb_sample_column<double, RT>(d, gp(d.alpha), tt, L - 1, cp(d.u)[0], &kblk);
-    if (threadIdx.x == 0) gp(d.path)[L - 1] = k;
-    __syncthreads();
-    tb_range<double, RT>(d, gp(d.alpha), tt, L - 2, 0, k, kblk, 0);
-}
-
-//===========================================================================
-// FP32 column-tiled forward
-
-struct Chunk {
-    int inst, c0, c1, b0;   // columns [c0, c1) of instance inst; block of c0
-    int first, last;        // first / last tile of its instance
-};
-
-// per-lane view of one block for the FP32 forward
-template <int RM>
-struct Lane32 {
-    int key;
-    int slot[RM], j1[RM], ln[RM], cb[RM], cs[RM];
-    float ei[RM], u[RM];
-    float tmc[MAXT - 1];
-    bool gend;
-};
-
-template <int RM>
-__device__ inline void lane32_setup(const DevInst &d, int b, Lane32<RM> &ls, float *t2, float *G)
-{
-    const int lane = threadIdx.x;
-    const int T1 = d.T - 1, TT = T1 * T1, N = d.N;
-    const int so = cp(d.soff)[b];
-    // the instance's slot table has d.RM <= RM slots per lane (the kernel is
-    // instantiated for the largest RM of the batch); the rest stay empty
-    const int IR = d.RM;
-    ls.key = gp(d.lkey)[(size_t)b * 64 + lane];
-    int knext = __shfl_down(ls.key, 1, 64);
-    ls.gend = (ls.key < T1) && (lane == 63 || knext != ls.key);
-#pragma unroll
-    for (int r = 0; r < RM; r++) {
-        int k = (r < IR) ? gp(d.lslot)[((size_t)b * 64 + lane) * IR + r] : -1;
-        ls.slot[r] = k;
-        ls.j1[r] = 0; ls.ln[r] = 0; ls.cb[r] = 0; ls.cs[r] = 1; ls.ei[r] = 0.f; ls.u[r] = 0.f;
-        if (k >= 0) {
-            int n = gp(d.snode)[so + k];
-            int lo = gp(d.age)[(size_t)b * N + n];
-            int hi = gp(d.nhi)[(size_t)b * N + n];
-            ls.j1[r] = gp(d.nfirst)[(size_t)b * N + n];
-            ls.ln[r] = hi - lo + 1;
-            ls.cs[r] = T1 - lo;
-            ls.cb[r] = tc_base(lo, T1) + (ls.key - lo);
-            ls.ei[r] = gp(d.einv32)[so + k];
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < MAXT - 1; a++)
-        ls.tmc[a] = (a < T1 && ls.key < T1) ? gp(d.tm32)[(size_t)b * TT + a * T1 + ls.key] : 0.f;
-    for (int x = lane; x < d.TC; x += 64) t2[x] = gp(d.tm2_32)[(size_t)b * d.TC + x];
-    for (int a = lane; a < 32; a += 64) G[a] = 0.f;
-}
-
-// per-time sums of the lanes' partials: segmented scan over lanes grouped by key
-__device__ inline void seg_reduce_store(float part, int key, bool gend, float *G)
-{
-    const int lane = threadIdx.x;
-    float v = part;
-    for (int off = 1; off < 64; off <<= 1) {
-        float vu = __shfl_up(v, off, 64);
-        int ku = __shfl_up(key, off, 64);
-        if (lane >= off && ku == key) v += vu;
-    }
-    if (gend) G[key] = v;
-}
-
-template <int RM>
-__global__ __launch_bounds__(64) void k_forward32(const DevInst *__restrict__ D,
-                                                  const Chunk *__restrict__ chunks, int mode,
-                                                  const float *__restrict__ snap, int sstride,
-                                                  int *__restrict__ flags, float tol_rel,
-                                                  float tol_abs)
-{
-    extern __shared__ float sm32[];
-    const Chunk ch = chunks[blockIdx.x];
-    const DevInst &d = D[ch.inst];
-    const int lane = threadIdx.x;
-    const int T1 = d.T - 1, Smax = d.Smax, L = d.L;
-    auto cls_a = cp(d.cls);
-    auto vemit = gp(d.vemit);
-    auto alpha = gp(d.alpha32);
-    auto norms = gp(d.norms);
-    float *A = sm32, *Bn = sm32 + Smax;
-    float *t2 = Bn + Smax;
-    float *G = t2 + d.TC;
-    Lane32<RM> ls;
-
-    int b = ch.b0;
-    int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-    int bstart = cp(d.bstart)[b], bend = bstart + cp(d.blen)[b];
-    long long aoff = cp(d.aoff)[b];
-    float sc = 1.f, tfb = 0.f;
-    int col = ch.c0;
-
-    // column c0 - 1 into A (uniform guess, or the true column in a fix-up pass)
-    if (col > 0) {
-        const int bp = (col - 1 >= bstart) ? b : b - 1;
-        const int Sp = cp(d.soff)[bp + 1] - cp(d.soff)[bp];
-        if (mode == 0) {
-            for (int k = lane; k < Sp; k += 64) A[k] = 1.f;
-            sc = 1.f / Sp;
-        } else {
-            for (int k = lane; k < Sp; k += 64) A[k] = snap[(size_t)blockIdx.x * sstride + k];
-            sc = 1.f;
-        }
-    }
-    lane32_setup<RM>(d, b, ls, t2, G);
-    wave_lds_sync();
-    if (col > 0 && col != bstart) {
-        float part = 0.f;
-#pragma unroll
-        for (int r = 0; r < RM; r++)
-            if (ls.slot[r] >= 0) part += A[ls.slot[r]];
-        seg_reduce_store(part, ls.key, ls.gend, G);
-        wave_lds_sync();
-        float t = 0.f;
-#pragma unroll
-        for (int a = 0; a < MAXT - 1; a++)
-            if (a < T1) t += ls.tmc[a] * G[a];
-        tfb = t * sc;
-    }
-    // next block's bounds, fetched ahead of need
-    int nb_S = 0, nb_so = 0, nb_len = 0;
-    long long nb_aoff = 0;
-    if (b + 1 < d.B) {
-        nb_so = cp(d.soff)[b + 1];
-        nb_S = cp(d.soff)[b + 2] - nb_so;
-        nb_len = cp(d.blen)[b + 1];
-        nb_aoff = cp(d.aoff)[b + 1];
-    }
-
-    int cls_next = cls_a[col];
-    int converged = -1;
-    for (; col < ch.c1; col++) {
-        if (col == bend) {
-            b++;
-            so = nb_so;
-            S = nb_S;
-            bstart = bend;
-            bend = bstart + nb_len;
-            aoff = nb_aoff;
-            lane32_setup<RM>(d, b, ls, t2, G);
-            if (b + 1 < d.B) {
-                nb_so = cp(d.soff)[b + 1];
-                nb_S = cp(d.soff)[b + 2] - nb_so;
-                nb_len = cp(d.blen)[b + 1];
-                nb_aoff = cp(d.aoff)[b + 1];
-            }
-            wave_lds_sync();
-        }
-        const int cls = cls_next;
-        if (col + 1 < ch.c1) cls_next = cls_a[col + 1];
-        const long long row = aoff + (long long)(col - bstart) * S;
-        float ov[RM];
-        if (mode == 1) {
-#pragma unroll
-            for (int r = 0; r < RM; r++) ov[r] = (ls.slot[r] >= 0) ? alpha[row + ls.slot[r]] : 0.f;
-        }
-        float part = 0.f;
-        const bool is_prior = (col == 0);
-        const bool is_switch = (col > 0 && col == bstart);
-        int rs = -1, csrc = -1, so1 = 0;
-        float prs = 0.f, pcs = 0.f;
-        AW_GLOBAL const int *ptr = nullptr;
-        if (is_switch) {
-            rs = cp(d.srcs)[2 * b];
-            csrc = cp(d.srcs)[2 * b + 1];
-            so1 = cp(d.soff)[b - 1];
-            ptr = gp(d.invptr) + so + b;
-            prs = (rs != -1) ? A[rs] * sc : 0.f;
-            pcs = (csrc != -1) ? A[csrc] * sc : 0.f;
-        }
-#pragma unroll
-        for (int r = 0; r < RM; r++) {
-            const int k = ls.slot[r];
-            if (k < 0) continue;
-            float v;
-            if (is_prior) {
-                v = (float)gp(d.prior)[k];
-            } else {
-                float e = (cls == -1) ? ls.ei[r]
-                        : (cls == -2) ? 1.f : (float)vemit[(size_t)cls * Smax + k];
-                if (is_switch) {
-                    float acc = 0.f;
-                    for (int x = ptr[k]; x < ptr[k + 1]; x++) {
-                        int j = gp(d.invidx)[so1 + x];
-                        acc += (A[j] * sc) * (float)gp(d.determp)[so1 + j];
-                    }
-                    float rr = (float)gp(d.recombrow)[so + k], cr = (float)gp(d.recoalrow)[so + k];
-                    if (rs != -1 && rr > 0.f) acc += prs * rr;
-                    if (csrc != -1 && cr > 0.f) acc += pcs * cr;
-                    v = acc * e;
-                } else {
-                    // four independent partial sums per pass keep the LDS
-                    // loads of a slot in flight together
-                    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-                    const float *cf = t2 + ls.cb[r];
-                    const float *src = A + ls.j1[r];
-                    const int st = ls.cs[r], ln = ls.ln[r];
-                    for (int x = 0; x < ln; x += 4) {
-                        float c0 = cf[x * st], c1 = cf[(x + 1) * st];
-                        float c2 = cf[(x + 2) * st], c3 = cf[(x + 3) * st];
-                        float v0 = src[x], v1 = src[x + 1], v2 = src[x + 2], v3 = src[x + 3];
-                        s0 += c0 * v0;
-                        s1 += (x + 1 < ln) ? c1 * v1 : 0.f;
-                        s2 += (x + 2 < ln) ? c2 * v2 : 0.f;
-                        s3 += (x + 3 < ln) ? c3 * v3 : 0.f;
-                    }
-                    v = (tfb + sc * ((s0 + s1) + (s2 + s3))) * e;
-                }
-            }
-            ls.u[r] = v;
-            Bn[k] = v;
-            part += v;
-        }
-        seg_reduce_store(part, ls.key, ls.gend, G);
-        wave_lds_sync();
-        float norm = 0.f;
-#pragma unroll
-        for (int a = 0; a < MAXT - 1; a++)
-            if (a < T1) norm += G[a];
-        float nsc;
-        double nrm;
-        if (is_prior) {
-            nsc = 1.f;
-            nrm = (L == 1) ? (double)norm : 1.0;
-        } else {
-            nsc = 1.f / norm;
-            nrm = (double)norm;
-        }
-        float t = 0.f;
-#pragma unroll
-        for (int a = 0; a < MAXT - 1; a++)
-            if (a < T1) t += ls.tmc[a] * G[a];
-        tfb = t * nsc;
         sc = nsc;
-        if (lane == 0) norms[col] = nrm;
-        if (mode == 1) {
-            bool ok = true;
-#pragma unroll
-            for (int r = 0; r < RM; r++) {
-                if (ls.slot[r] < 0) continue;
-                float nv = ls.u[r] * sc;
-                if (!(fabsf(nv - ov[r]) <= tol_rel * fabsf(ov[r]) + tol_abs)) ok = false;
-            }
-            if (__all(ok)) {
-                converged = col;
-                break;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RM; r++)
-            if (ls.slot[r] >= 0) alpha[row + ls.slot[r]] = ls.u[r] * sc;
-        float *tmp = A; A = Bn; Bn = tmp;
     }
-    if (mode == 1 && lane == 0) flags[blockIdx.x] = converged;
 }
 
-// boundary column of every tile after the first: its previous column
-__global__ __launch_bounds__(64) void k_fw_snapshot(const DevInst *__restrict__ D,
-                                                    const Chunk *__restrict__ chunks,
-                                                    float *__restrict__ snap, int sstride)
+template <class Real, int R, int NT1>
+__global__ __launch_bounds__(64) void k_fw_lane(const DevInst *__restrict__ D,
+                                                const int *__restrict__ list)
 {
-    const Chunk ch = chunks[blockIdx.x];
-    if (ch.first) return;
-    const DevInst &d = D[ch.inst];
-    const int col = ch.c0 - 1;
-    const int b = find_owner(cp(d.bstart), d.B, col);
-    const int S = cp(d.soff)[b + 1] - cp(d.soff)[b];
-    const long long row = cp(d.aoff)[b] + (long long)(col - cp(d.bstart)[b]) * S;
-    for (int k = threadIdx.x; k < S; k += 64) snap[(size_t)blockIdx.x * sstride + k] = gp(d.alpha32)[row + k];
-}
-
-//===========================================================================
-// FP32 column-tiled traceback (FP64 weights and CDF)
-
-template <int RT>
-__global__ __launch_bounds__(64) void k_traceback32(const DevInst *__restrict__ D,
-                                                    const Chunk *__restrict__ chunks, int mode,
-                                                    const int *__restrict__ snap,
-                                                    int *__restrict__ flags)
-{
-    extern __shared__ double dyn[];
-    const Chunk ch = chunks[blockIdx.x];
-    const DevInst &d = D[ch.inst];
-    TbTables tt = tb_layout(dyn, d);
-    const int L = d.L;
-    int knext, kblk, hi = ch.c1 - 1;
-    if (ch.last) {
-        knext = tb_sample_column<float, RT>(d, gp(d.alpha32), tt, L - 1, cp(d.u)[0], &kblk);
-        if (threadIdx.x == 0) gp(d.path)[L - 1] = knext;
-        __syncthreads();
-        hi = L - 2;
-    } else if (mode == 0) {
-        // speculative state at column c1: sample that column alone
-        knext = tb_sample_column<float, RT>(d, gp(d.alpha32), tt, ch.c1, cp(d.u)[L - 1 - ch.c1], &kblk);
-    } else {
-        knext = snap[blockIdx.x];
-        kblk = find_owner(cp(d.bstart), d.B, ch.c1);
-    }
-    int coal = tb_range<float, RT>(d, gp(d.alpha32), tt, hi, ch.c0, knext, kblk, mode);
-    if (mode == 1 && threadIdx.x == 0) flags[blockIdx.x] = coal;
-}
-
-__global__ __launch_bounds__(64) void k_tb_snapshot(const DevInst *__restrict__ D,
-                                                    const Chunk *__restrict__ chunks,
-                                                    int *__restrict__ snap)
-{
-    const Chunk ch = chunks[blockIdx.x];
-    if (ch.last || threadIdx.x != 0) return;
-    snap[blockIdx.x] = D[ch.inst].path[ch.c1];
+    extern __shared__ double fw_dyn[];
+    fw_instance<Real, R, NT1>(D[list[blockIdx.x]], (Real *)fw_dyn);
 }
 
 // log-likelihood: sum of log column normalisers
@@ -1194,11 +800,286 @@ __global__ __launch_bounds__(256) void k_lnl_b(const DevInst *__restrict__ D)
     if (threadIdx.x == 0) *d.lnl = part[0];
 }
 
-// FP64 copy of an FP32 table
-__global__ void k_widen(const float *__restrict__ src, double *__restrict__ dst, long long n)
+//===========================================================================
+// stochastic traceback (stochastic_traceback, sample_thread.cpp:522-569;
+// sample_hmm_posterior_step :506-519; sample, common.h:272-290)
+
+struct Chunk {
+    int inst, c0, c1;        // columns [c0, c1) of instance inst
+    int first, last;         // first / last tile of its instance
+};
+
+template <int R>
+struct TbLane {
+    int ti[R];
+    int off1[R];   // a * (T-1): other-branch entry P(a -> b) at off1 + b
+    int off2[R];   // same-branch entry P(a -> b) at off2 + b
+    int dt[R];     // deterministic switch target into the next block
+    double dp[R];  // and its weight
+};
+
+// LDS tables of block b: the (a,b) table followed by the (c,a,b) table,
+// copied with 16 loads per lane in flight
+template <int R>
+__device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R> &tl, double *tab)
 {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (double)src[i];
+    constexpr int RW = 64 * R;
+    const int lane = threadIdx.x, T1 = d.T - 1, TT = T1 * T1;
+    const AW_GLOBAL int *info = gp(d.linfo) + (size_t)b * RW;
+    // switch into block b+1 from this block's states (the last block
+    // re-reads its own entries, never used)
+    const int bn = min(b + 1, d.B - 1);
+    const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)bn * RW;
+    const AW_GLOBAL double *dpn = gp(d.lcoef) + ((size_t)bn * NCOEF + CDP) * RW;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        tl.ti[r] = info[r * 64 + lane];
+        tl.dt[r] = detn[r * 64 + lane];
+        tl.dp[r] = dpn[r * 64 + lane];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int a = tl.ti[r] & INFO_TIME, c = info_age(tl.ti[r]);
+        tl.off1[r] = a * T1;
+        tl.off2[r] = TT + tc_base(c, T1) + (a - c) * (T1 - c) - c;
+    }
+    const AW_GLOBAL double *tm = gp(d.tm) + (size_t)b * TT;
+    const AW_GLOBAL double *tm2 = gp(d.tm2s) + (size_t)b * d.TC;
+    const int n = TT + d.TC;
+    constexpr int CP = 16;
+    for (int base = 0; base < n; base += 64 * CP) {
+        double t[CP];
+#pragma unroll
+        for (int k = 0; k < CP; k++) {
+            const int x = min(base + k * 64 + lane, n - 1);
+            const AW_GLOBAL double *src = (x < TT) ? tm + x : tm2 + (x - TT);
+            t[k] = *src;
+        }
+#pragma unroll
+        for (int k = 0; k < CP; k++) {
+            const int x = base + k * 64 + lane;
+            if (x < n) tab[x] = t[k];
+        }
+    }
+    lds_order();
+}
+
+// First index k of the lane-run weights (lane l holds states l*R..l*R+R-1)
+// whose cumulative sum reaches u * total; S-1 if none does.  Returns k and
+// the info word of state k in kinfo.
+template <int R>
+__device__ __forceinline__ int lane_sample(const double (&w)[R], const int (&ti)[R], int S, double u,
+                                  int &kinfo)
+{
+    const int lane = threadIdx.x;
+    double loc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) loc += w[r];
+    const double inc = wave_incl_scan(loc);
+    const double excl = dpp_r<DPP_WAVE_SHR1>(inc);
+    const double total = readlane_d(inc, 63);
+    const double pick = u * total;
+    const bool has = lane * R < S;
+    const unsigned long long m = __ballot(has && excl + loc >= pick);
+    const int owner = m ? (int)__ffsll((long long)m) - 1 : (S - 1) / R;
+    const int nloc = min(R, S - lane * R);
+    int rsel = nloc - 1;
+    if (m) {
+        double p = excl;
+        bool found = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!found && r < nloc) {
+                p += w[r];
+                if (p >= pick) {
+                    found = true;
+                    rsel = r;
+                }
+            }
+        }
+    }
+    // select ti[rsel] with masks: a compare-and-branch form lets the
+    // optimiser rewrite it as a dynamically indexed load, which puts ti in
+    // scratch memory
+    int inf = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) inf |= ti[r] & -(int)(r == rsel);
+    kinfo = __builtin_amdgcn_readlane(inf, owner);
+    return __builtin_amdgcn_readlane(lane * R + rsel, owner);
+}
+
+// forward-table rows staged per traceback batch
+template <class Real, int R>
+constexpr int tb_batch()
+{
+    return (64 / (R * (int)(sizeof(Real) / 4))) > 16 ? 16
+         : (64 / (R * (int)(sizeof(Real) / 4))) < 2 ? 2
+                                                     : (64 / (R * (int)(sizeof(Real) / 4)));
+}
+
+template <class Real, int R>
+__device__ __forceinline__ void row_load(AW_GLOBAL const Real *alpha, int col, Real (&rv)[R])
+{
+    constexpr int RW = 64 * R;
+    AW_GLOBAL const Real *row = alpha + (size_t)col * RW;
+#pragma unroll
+    for (int r = 0; r < R; r++) rv[r] = row[r * 64 + threadIdx.x];
+}
+
+// Sample the tile's columns [lo, hi] given the state knext (info kinfo) at
+// column hi+1 in block kblk.  mode 1 stops at the first column whose sample
+// equals the stored path (the chains have coalesced) and returns it; else
+// returns -1.  The uniform of column col is u[L-1-col].
+//
+// Rows come through LDS in batches of BT columns, double-buffered: the
+// loads of the next batch are issued when a batch starts and written to
+// LDS when it ends, so the column loop itself never waits on HBM (vmcnt
+// orders loads behind stores, so a per-column prefetch would not hide it).
+template <class Real, int R>
+__device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *rows, int hi, int lo,
+                                        int knext, int kinfo, int kblk, int mode)
+{
+    constexpr int RW = 64 * R;
+    constexpr int BT = tb_batch<Real, R>();
+    const int lane = threadIdx.x, L = d.L;
+    AW_GLOBAL const Real *alpha = (AW_GLOBAL const Real *)d.alpha;
+    auto path = gp(d.path);
+    auto spath = cp(d.path);   // the stored path of an earlier pass (mode 1)
+    auto u = cp(d.u);
+    if (hi < lo) return -1;
+    int bc = find_owner(cp(d.bstart), d.B, hi);
+    int bc_start = cp(d.bstart)[bc];
+    int S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
+    TbLane<R> tl;
+    tb_load_block<R>(d, bc, tl, tab);
+    Real nxt[BT][R];
+#pragma unroll
+    for (int k = 0; k < BT; k++) row_load<Real, R>(alpha, max(hi - k, lo), nxt[k]);
+#pragma unroll
+    for (int k = 0; k < BT; k++)
+#pragma unroll
+        for (int r = 0; r < R; r++) rows[k * RW + r * 64 + lane] = nxt[k][r];
+    int buf = 0;
+    for (int top = hi; top >= lo; top -= BT) {
+        const int bot = max(top - BT + 1, lo);
+#pragma unroll
+        for (int k = 0; k < BT; k++) row_load<Real, R>(alpha, max(top - BT - k, lo), nxt[k]);
+        const Real *rb = rows + buf * BT * RW;
+        for (int col = top; col >= bot; col--) {
+            const int oldp = (mode == 1) ? spath[col] : -1;
+            const Real *row = rb + (top - col) * RW;
+            double w[R];
+            if (bc != kblk) {
+                // col is the last column of block bc: switch into kblk = bc+1
+                // (TransMatrixSwitch::get, trans.h:201-209)
+                const int rs = cp(d.srcs)[2 * kblk], cs = cp(d.srcs)[2 * kblk + 1];
+                const int pk = lane_pos(knext, R);
+                auto cn = cp(d.lcoef) + (size_t)kblk * NCOEF * RW;
+                const double rrk = cn[CRR * RW + pk], crk = cn[CCR * RW + pk];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int s = lane * R + r;
+                    const double tr = (s == cs) ? crk : (s == rs) ? rrk
+                                    : (tl.dt[r] == knext ? tl.dp[r] : 0.0);
+                    w[r] = (tl.ti[r] & INFO_VALID) ? (double)row[r * 64 + lane] * tr : 0.0;
+                }
+            } else {
+                const int nk = info_node(kinfo), bk = kinfo & INFO_TIME;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool same = info_node(tl.ti[r]) == nk;
+                    const double tr = tab[(same ? tl.off2[r] : tl.off1[r]) + bk];
+                    w[r] = (tl.ti[r] & INFO_VALID) ? (double)row[r * 64 + lane] * tr : 0.0;
+                }
+            }
+            knext = lane_sample<R>(w, tl.ti, S, u[L - 1 - col], kinfo);
+            kblk = bc;
+            if (mode == 1 && oldp == knext) return col;
+            if (lane == 0) path[col] = knext;
+            if (col - 1 < bc_start && col - 1 >= lo) {
+                bc--;
+                bc_start = cp(d.bstart)[bc];
+                S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
+                tb_load_block<R>(d, bc, tl, tab);
+            }
+        }
+        buf ^= 1;
+        Real *wb = rows + buf * BT * RW;
+#pragma unroll
+        for (int k = 0; k < BT; k++)
+#pragma unroll
+            for (int r = 0; r < R; r++) wb[k * RW + r * 64 + lane] = nxt[k][r];
+        lds_order();
+    }
+    return -1;
+}
+
+// state sampled from column col alone (the last column, or the speculative
+// start of a tile)
+template <class Real, int R>
+__device__ __forceinline__ int tb_column(const DevInst &d, int col, double uval, int &kinfo, int &blk)
+{
+    constexpr int RW = 64 * R;
+    AW_GLOBAL const Real *alpha = (AW_GLOBAL const Real *)d.alpha;
+    const int b = find_owner(cp(d.bstart), d.B, col);
+    const int S = cp(d.soff)[b + 1] - cp(d.soff)[b];
+    const AW_GLOBAL int *info = gp(d.linfo) + (size_t)b * RW;
+    Real rv[R];
+    row_load<Real, R>(alpha, col, rv);
+    int ti[R];
+    double w[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        ti[r] = info[r * 64 + threadIdx.x];
+        w[r] = (ti[r] & INFO_VALID) ? (double)rv[r] : 0.0;
+    }
+    blk = b;
+    return lane_sample<R>(w, ti, S, uval, kinfo);
+}
+
+template <class Real, int R>
+__device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mode, const int *snap, double *tab)
+{
+    Real *rows = (Real *)(tab + (d.T - 1) * (d.T - 1) + d.TC);
+    constexpr int RW = 64 * R;
+    const int L = d.L;
+    int knext, kinfo, kblk, hi;
+    if (ch.last) {
+        knext = tb_column<Real, R>(d, L - 1, cp(d.u)[0], kinfo, kblk);
+        if (threadIdx.x == 0) gp(d.path)[L - 1] = knext;
+        hi = L - 2;
+    } else if (mode == 0) {
+        knext = tb_column<Real, R>(d, ch.c1, cp(d.u)[L - 1 - ch.c1], kinfo, kblk);
+        hi = ch.c1 - 1;
+    } else {
+        knext = snap[blockIdx.x];
+        kblk = find_owner(cp(d.bstart), d.B, ch.c1);
+        kinfo = gp(d.linfo)[(size_t)kblk * RW + lane_pos(knext, R)];
+        hi = ch.c1 - 1;
+    }
+    return tb_range<Real, R>(d, tab, rows, hi, ch.c0, knext, kinfo, kblk, mode);
+}
+
+template <class Real, int R>
+__global__ __launch_bounds__(64) void k_tb_lane(const DevInst *__restrict__ D,
+                                                const Chunk *__restrict__ chunks, int mode,
+                                                const int *__restrict__ snap,
+                                                int *__restrict__ flags)
+{
+    extern __shared__ double tb_dyn[];
+    const Chunk ch = chunks[blockIdx.x];
+    const int res = tb_tile<Real, R>(D[ch.inst], ch, mode, snap, tb_dyn);
+    if (mode == 1 && threadIdx.x == 0) flags[blockIdx.x] = res;
+}
+
+// boundary state of every tile before the last: the path at column c1
+__global__ __launch_bounds__(64) void k_tb_snapshot(const DevInst *__restrict__ D,
+                                                    const Chunk *__restrict__ chunks,
+                                                    int *__restrict__ snap)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    if (ch.last || threadIdx.x != 0) return;
+    snap[blockIdx.x] = D[ch.inst].path[ch.c1];
 }
 
 //===========================================================================
@@ -1206,10 +1087,16 @@ __global__ void k_widen(const float *__restrict__ src, double *__restrict__ dst,
 
 namespace {
 
+// supported states-per-lane counts (one kernel instantiation each)
+constexpr int NRV = 7;
+constexpr int RVALS[NRV] = {1, 2, 4, 6, 8, 12, 16};
+
 struct Ctx {
     bool init = false;
     int dev = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[NRV];        // one per R group, all at the default priority
+    hipEvent_t fork, join[NRV];
     hipDeviceProp_t prop{};
     hipEvent_t ev[8];
 };
@@ -1227,6 +1114,9 @@ int ctx_init()
     HIP_OK(hipGetDevice(&c.dev));
     HIP_OK(hipGetDeviceProperties(&c.prop, c.dev));
     HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    for (auto &st : c.side) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
+    for (auto &e : c.join) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : c.ev) HIP_OK(hipEventCreate(&e));
     c.init = true;
     return 0;
@@ -1267,6 +1157,88 @@ struct Plan {
     }
 };
 
+// smallest supported states-per-lane count covering Smax states
+int lanes_per_state_r(int Smax)
+{
+    for (int R : RVALS)
+        if (64 * R >= Smax) return R;
+    return -1;
+}
+
+int r_index(int R)
+{
+    for (int g = 0; g < NRV; g++)
+        if (RVALS[g] == R) return g;
+    return -1;
+}
+
+// Launch per-R work on the side streams, concurrently, joined back into the
+// main stream.  cnt[g] items of group g start at offset off[g].
+template <class F>
+int launch_groups(Ctx &c, const int *cnt, const int *off, F fn)
+{
+    HIP_OK(hipEventRecord(c.fork, c.stream));
+    for (int g = 0; g < NRV; g++) {
+        if (!cnt[g]) continue;
+        HIP_OK(hipStreamWaitEvent(c.side[g], c.fork, 0));
+        if (fn(g, off[g], cnt[g], c.side[g])) return 1;
+        HIP_OK(hipEventRecord(c.join[g], c.side[g]));
+        HIP_OK(hipStreamWaitEvent(c.stream, c.join[g], 0));
+    }
+    return 0;
+}
+
+template <class Real, int R>
+int launch_fw(const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st, int T1max)
+{
+    auto kern = (T1max <= 19) ? k_fw_lane<Real, R, 19> : k_fw_lane<Real, R, MAXT - 1>;
+    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, list);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+template <class Real, int R>
+int launch_tb(const DevInst *dD, const Chunk *chunks, int cnt, int mode, const int *snap,
+              int *flags, size_t sh, hipStream_t st)
+{
+    auto kern = k_tb_lane<Real, R>;
+    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, chunks, mode, snap, flags);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+template <class Real>
+int launch_fw_r(int g, const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st,
+                int T1max)
+{
+    switch (RVALS[g]) {
+    case 1: return launch_fw<Real, 1>(dD, list, cnt, sh, st, T1max);
+    case 2: return launch_fw<Real, 2>(dD, list, cnt, sh, st, T1max);
+    case 4: return launch_fw<Real, 4>(dD, list, cnt, sh, st, T1max);
+    case 6: return launch_fw<Real, 6>(dD, list, cnt, sh, st, T1max);
+    case 8: return launch_fw<Real, 8>(dD, list, cnt, sh, st, T1max);
+    case 12: return launch_fw<Real, 12>(dD, list, cnt, sh, st, T1max);
+    default: return launch_fw<Real, 16>(dD, list, cnt, sh, st, T1max);
+    }
+}
+
+template <class Real>
+int launch_tb_r(int g, const DevInst *dD, const Chunk *chunks, int cnt, int mode,
+                const int *snap, int *flags, size_t sh, hipStream_t st)
+{
+    switch (RVALS[g]) {
+    case 1: return launch_tb<Real, 1>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    case 2: return launch_tb<Real, 2>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    case 4: return launch_tb<Real, 4>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    case 6: return launch_tb<Real, 6>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    case 8: return launch_tb<Real, 8>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    case 12: return launch_tb<Real, 12>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    default: return launch_tb<Real, 16>(dD, chunks, cnt, mode, snap, flags, sh, st);
+    }
+}
+
 }  // namespace
 
 struct Batch::Impl {
@@ -1274,19 +1246,16 @@ struct Batch::Impl {
     std::vector<std::vector<double>> us, priors;
     std::vector<std::vector<int>> Sv;
     std::vector<std::vector<int>> soff;
-    std::vector<std::vector<long long>> aoff;
     std::vector<DevInst> h;
     std::vector<long long> W;
     std::vector<double> lnl;
     std::vector<int> nvar;
-    std::vector<int> RMv;
     std::vector<Chunk> chunks;
     DevBuf in, work, tab, aux, vbuf;
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
     int *d_blkoff = nullptr, *d_voff = nullptr, *d_nvar = nullptr, *d_list = nullptr;
-    int *d_flags = nullptr, *d_ovf = nullptr, *d_tsnap = nullptr;
-    float *d_fsnap = nullptr;
+    int *d_flags = nullptr, *d_tsnap = nullptr;
     Chunk *d_chunks = nullptr;
     bool uploaded = false;
     Precision prec_done = Precision::FP64;
@@ -1332,7 +1301,6 @@ int Batch::upload()
     const int n = (int)m.hosts.size();
     m.h.assign(n, DevInst{});
     m.soff.assign(n, {});
-    m.aoff.assign(n, {});
     m.lnl.assign(n, 0.0);
     m.nvar.assign(n, 0);
     for (int pass = 0; pass < 2; pass++) {
@@ -1350,17 +1318,20 @@ int Batch::upload()
                     return 1;
                 }
                 std::vector<int> &so = m.soff[i];
-                std::vector<long long> &ao = m.aoff[i];
                 so.assign(B + 1, 0);
-                ao.assign(B + 1, 0);
                 int Smax = 1;
                 for (int b = 0; b < B; b++) {
                     so[b + 1] = so[b] + m.Sv[i][b];
-                    ao[b + 1] = ao[b] + (long long)m.Sv[i][b] * ft.len[b];
                     Smax = std::max(Smax, m.Sv[i][b]);
                 }
+                const int R = lanes_per_state_r(Smax);
+                if (R < 0) {
+                    fprintf(stderr, "argweaver_amd: %d states per block exceed the 1024 of one wave\n",
+                            Smax);
+                    return 1;
+                }
                 d.T = T; d.N = N; d.B = B; d.L = L; d.nrows = (int)hi.rows.size();
-                d.Smax = Smax; d.TC = tc_base(T - 1, T - 1); d.W = m.W[i];
+                d.Smax = Smax; d.TC = tc_base(T - 1, T - 1); d.W = m.W[i]; d.R = R;
             }
             d.model = p.take<Model>(1);
             d.parent = p.take<int>((size_t)B * N);
@@ -1372,7 +1343,6 @@ int Batch::upload()
             d.bstart = p.take<int>(B);
             d.blen = p.take<int>(B);
             d.soff = p.take<int>(B + 1);
-            d.aoff = p.take<long long>(B + 1);
             d.seq = p.take<char>((size_t)d.nrows * L);
             d.u = p.take<double>(L);
             if (pass == 1) {
@@ -1387,7 +1357,6 @@ int Batch::upload()
                 HIP_OK(hipMemcpyAsync((void *)d.bstart, ft.start.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync((void *)d.blen, ft.len.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync((void *)d.soff, m.soff[i].data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice, s));
-                HIP_OK(hipMemcpyAsync((void *)d.aoff, m.aoff[i].data(), sizeof(long long) * (B + 1), hipMemcpyHostToDevice, s));
                 for (int r = 0; r < d.nrows; r++)
                     HIP_OK(hipMemcpyAsync((char *)d.seq + (size_t)r * L, hi.rows[r], L, hipMemcpyHostToDevice, s));
                 if (!m.us[i].empty())
@@ -1403,21 +1372,6 @@ int Batch::upload()
     return 0;
 }
 
-static size_t lds_fw64(const DevInst &d)
-{
-    return sizeof(double) * (2 * (size_t)d.Smax + d.TC + 32) + sizeof(int) * ((size_t)d.Smax + 33);
-}
-static size_t lds_tb(const DevInst &d)
-{
-    const int T1 = d.T - 1;
-    return sizeof(double) * ((size_t)d.Smax + T1 * T1 + d.TC) + sizeof(int) * (2 * (size_t)d.Smax);
-}
-static size_t lds_fw32(const DevInst &d)
-{
-    // + 4*(T-1) floats: the 4-way gather reads up to three strides past a run
-    return sizeof(float) * (2 * (size_t)d.Smax + d.TC + 32 + 4 * (d.T - 1) + 8);
-}
-
 int Batch::run(const BatchOptions &opt, RunStats *st)
 {
     if (ctx_init()) return 1;
@@ -1427,115 +1381,79 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     const int n = (int)m.hosts.size();
     hipStream_t s = c.stream;
     const bool fp32 = opt.prec == Precision::FP32;
+    const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
 
-    // classification and variant numbering: one host sync for the counts
     std::vector<long long> coloff(n + 1, 0);
     std::vector<int> blkoff(n + 1, 0);
     for (int i = 0; i < n; i++) {
         coloff[i + 1] = coloff[i] + m.h[i].L;
         blkoff[i + 1] = blkoff[i] + m.h[i].B;
     }
-    // work buffers, first pass sized without the variant rows
+    // per-instance work arrays
     for (int pass = 0; pass < 2; pass++) {
         Plan p;
         p.base = pass ? m.work.p : nullptr;
         for (int i = 0; i < n; i++) {
             DevInst &d = m.h[i];
-            const int B = d.B, N = d.N, L = d.L, T = d.T, T1 = T - 1;
+            const int B = d.B, N = d.N, L = d.L, T1 = d.T - 1, RW = 64 * d.R;
             const int Stot = m.soff[i][B];
             d.vflag = p.take<int>(L);
             d.cls = p.take<int>(L);
             d.vcol = p.take<int>(L);
             d.snode = p.take<int>(Stot);
             d.stime = p.take<int>(Stot);
-            d.nfirst = p.take<int>((size_t)B * N);
-            d.nhi = p.take<int>((size_t)B * N);
-            d.tperm = p.take<int>(Stot);
-            d.tstart = p.take<int>((size_t)B * (T + 1));
-            d.tv = p.take<double>((size_t)B * 9 * T);
-            d.einv = p.take<double>(Stot);
             d.eterm = p.take<double>((size_t)Stot * 6);
-            d.determ = p.take<int>(Stot);
-            d.determp = p.take<double>(Stot);
-            d.recombrow = p.take<double>(Stot);
-            d.recoalrow = p.take<double>(Stot);
-            d.srcs = p.take<int>(2 * (size_t)B);
-            d.invptr = p.take<int>((size_t)Stot + B);
-            d.invidx = p.take<int>(Stot);
-            d.prior = p.take<double>(d.Smax);
             d.order = p.take<int>((size_t)B * N);
             d.roots = p.take<int>(B);
             d.nmut = p.take<double>((size_t)B * N * 2);
+            d.tm = p.take<double>((size_t)B * T1 * T1);
+            d.tm2s = p.take<double>((size_t)B * d.TC);
+            d.srcs = p.take<int>(2 * (size_t)B);
+            d.linfo = p.take<int>((size_t)B * RW);
+            d.lcoef = p.take<double>((size_t)B * NCOEF * RW);
+            d.ldet = p.take<int>((size_t)B * RW);
+            d.lprior = p.take<double>(RW);
             d.norms = p.take<double>(L);
             d.path = p.take<int>(L);
             d.lnl = p.take<double>(1);
-            (void)T1;
         }
         if (pass == 0 && m.work.ensure(p.used + 4096)) return 1;
     }
-    // transition tables and the forward table
-    std::vector<int> RMv(n, 8);
+    // forward tables
     for (int pass = 0; pass < 2; pass++) {
         Plan p;
         p.base = pass ? m.tab.p : nullptr;
         for (int i = 0; i < n; i++) {
             DevInst &d = m.h[i];
-            const int B = d.B, T1 = d.T - 1;
-            const int Stot = m.soff[i][B];
-            d.tm = p.take<double>((size_t)B * T1 * T1);
-            d.tm2s = p.take<double>((size_t)B * d.TC);
-            if (fp32) {
-                // worst case slots per lane: all states in the largest bin
-                // sum_a ceil(cnt_a/R) <= S/R + T1 fits 64 lanes when R >= S/(64-T1)
-                int RM = 8;
-                while (RM < 64 && (long long)RM * (64 - T1) < d.Smax) RM *= 2;
-                RMv[i] = RM;
-                d.RM = RM;
-                d.tm2 = p.take<double>((size_t)B * d.TC);
-                d.tm32 = p.take<float>((size_t)B * T1 * T1);
-                d.tm2_32 = p.take<float>((size_t)B * d.TC);
-                d.einv32 = p.take<float>(Stot);
-                d.lslot = p.take<int>((size_t)B * 64 * RM);
-                d.lkey = p.take<int>((size_t)B * 64);
-                d.alpha32 = p.take<float>(d.W);
-                d.alpha = nullptr;
-            } else {
-                d.tm2 = p.take<double>((size_t)B * d.TC);
-                d.alpha = p.take<double>(d.W);
-                d.alpha32 = nullptr;
-            }
+            d.alpha = p.take<char>((size_t)d.L * 64 * d.R * rsz);
         }
         if (pass == 0 && m.tab.ensure(p.used + 4096)) return 1;
     }
-    // auxiliary arrays
+    // traceback tiles and auxiliary arrays
     {
-        size_t nchunk_max = 0;
         std::vector<Chunk> &chs = m.chunks;
         chs.clear();
-        for (int i = 0; i < n; i++) {
+        std::vector<int> byr(n);
+        std::iota(byr.begin(), byr.end(), 0);
+        std::stable_sort(byr.begin(), byr.end(),
+                         [&](int a, int b) { return m.h[a].R < m.h[b].R; });
+        for (int i : byr) {
             const DevInst &d = m.h[i];
-            const int tile = (fp32 && opt.chunk_cols > 0) ? opt.chunk_cols : d.L;
+            const int tile = opt.tile_cols > 0 ? opt.tile_cols : d.L;
             const int nt = (d.L + tile - 1) / tile;
-            const std::vector<int> &bs = m.hosts[i].ft.start;
-            int bcur = 0;
             for (int t = 0; t < nt; t++) {
                 Chunk k;
                 k.inst = i;
                 k.c0 = t * tile;
                 k.c1 = std::min(d.L, (t + 1) * tile);
-                while (bcur + 1 < d.B && bs[bcur + 1] <= k.c0) bcur++;
-                k.b0 = bcur;
                 k.first = (t == 0);
                 k.last = (t == nt - 1);
                 chs.push_back(k);
             }
         }
-        nchunk_max = chs.size();
-        int Smax_all = 1;
-        for (auto &d : m.h) Smax_all = std::max(Smax_all, d.Smax);
-        Plan p;
+        const size_t nch = chs.size();
         for (int pass = 0; pass < 2; pass++) {
-            p = Plan();
+            Plan p;
             p.base = pass ? m.aux.p : nullptr;
             m.dD = p.take<DevInst>(n);
             m.d_coloff = p.take<long long>(n + 1);
@@ -1543,11 +1461,9 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             m.d_voff = p.take<int>(n + 1);
             m.d_nvar = p.take<int>(n);
             m.d_list = p.take<int>(n);
-            m.d_flags = p.take<int>(nchunk_max);
-            m.d_ovf = p.take<int>(1);
-            m.d_tsnap = p.take<int>(nchunk_max);
-            m.d_chunks = p.take<Chunk>(nchunk_max);
-            m.d_fsnap = p.take<float>(fp32 ? nchunk_max * (size_t)Smax_all : 1);
+            m.d_flags = p.take<int>(nch);
+            m.d_tsnap = p.take<int>(nch);
+            m.d_chunks = p.take<Chunk>(nch);
             if (pass == 0 && m.aux.ensure(p.used + 4096)) return 1;
         }
     }
@@ -1565,18 +1481,15 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
 
     // variant rows, allocated now that the counts are known
     std::vector<int> voff(n + 1, 0);
-    {
+    for (int pass = 0; pass < 2; pass++) {
         Plan p;
-        for (int pass = 0; pass < 2; pass++) {
-            p = Plan();
-            p.base = pass ? m.vbuf.p : nullptr;
-            for (int i = 0; i < n; i++) {
-                m.h[i].nvar = m.nvar[i];
-                m.h[i].vemit = p.take<double>((size_t)std::max(m.nvar[i], 1) * m.h[i].Smax);
-                voff[i + 1] = voff[i] + m.nvar[i];
-            }
-            if (pass == 0 && m.vbuf.ensure(p.used + 4096)) return 1;
+        p.base = pass ? m.vbuf.p : nullptr;
+        for (int i = 0; i < n; i++) {
+            m.h[i].nvar = m.nvar[i];
+            m.h[i].vemit = p.take<double>((size_t)std::max(m.nvar[i], 1) * 64 * m.h[i].R);
+            voff[i + 1] = voff[i] + m.nvar[i];
         }
+        if (pass == 0 && m.vbuf.ensure(p.used + 4096)) return 1;
     }
     HIP_OK(hipMemcpyAsync(m.d_voff, voff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(m.dD, m.h.data(), sizeof(DevInst) * n, hipMemcpyHostToDevice, s));
@@ -1584,161 +1497,89 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     (void)hipEventRecord(c.ev[1], s);
     hipLaunchKernelGGL(k_prep_b, dim3(blkoff[n]), dim3(256), 0, s, m.dD, m.d_blkoff, n);
     HIP_OK(hipGetLastError());
-    for (int i = 0; i < n; i++)
-        if (!m.priors[i].empty())
-            HIP_OK(hipMemcpyAsync(m.h[i].prior, m.priors[i].data(), sizeof(double) * m.priors[i].size(),
-                                  hipMemcpyHostToDevice, s));
-    if (fp32) {
-        HIP_OK(hipMemsetAsync(m.d_ovf, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_prep32_b, dim3(blkoff[n]), dim3(64), 0, s, m.dD, m.d_blkoff, n, m.d_ovf);
-        HIP_OK(hipGetLastError());
+    // caller-given first-column priors, in the lane-run layout
+    std::vector<std::vector<double>> lpri(n);
+    for (int i = 0; i < n; i++) {
+        if (m.priors[i].empty()) continue;
+        const int R = m.h[i].R;
+        lpri[i].assign(64 * R, 0.0);
+        for (size_t k = 0; k < m.priors[i].size(); k++) lpri[i][lane_pos((int)k, R)] = m.priors[i][k];
+        HIP_OK(hipMemcpyAsync(m.h[i].lprior, lpri[i].data(), sizeof(double) * lpri[i].size(),
+                              hipMemcpyHostToDevice, s));
     }
     (void)hipEventRecord(c.ev[2], s);
     if (voff[n] > 0) hipLaunchKernelGGL(k_var_emit_b, dim3(voff[n]), dim3(64), 0, s, m.dD, m.d_voff, n);
     HIP_OK(hipGetLastError());
     (void)hipEventRecord(c.ev[3], s);
 
+    // forward: one wave per instance, grouped by R (one kernel per group on
+    // its own stream), longest first inside a group
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (m.h[a].R != m.h[b].R) return m.h[a].R < m.h[b].R;
+        return m.h[a].L > m.h[b].L;
+    });
+    int gcnt[NRV] = {0}, goff[NRV] = {0};
+    for (int i = 0; i < n; i++) gcnt[r_index(m.h[i].R)]++;
+    for (int g = 1; g < NRV; g++) goff[g] = goff[g - 1] + gcnt[g - 1];
+    HIP_OK(hipMemcpyAsync(m.d_list, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    size_t shf[NRV] = {0}, sht = 0;
+    int t1max[NRV] = {0};
+    for (int i = 0; i < n; i++) {
+        const DevInst &d = m.h[i];
+        const int T1 = d.T - 1;
+        t1max[r_index(d.R)] = std::max(t1max[r_index(d.R)], T1);
+        size_t &x = shf[r_index(d.R)];
+        const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
+        x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R));
+        // (a,b) and (c,a,b) tables, then two row batches
+        const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
+        sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
+    }
+    if (launch_groups(c, gcnt, goff, [&](int g, int off, int cnt, hipStream_t st) {
+            return fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
+                        : launch_fw_r<double>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g]);
+        }))
+        return 1;
+    (void)hipEventRecord(c.ev[4], s);
+    hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD);
+    HIP_OK(hipGetLastError());
+
     RunStats rs;
-    if (!fp32) {
-        // group instances by the slot count of the plain forward
-        std::vector<int> lists[6];
-        for (int i = 0; i < n; i++) {
-            const int S = m.h[i].Smax;
-            int g = S <= 256 ? 0 : S <= 512 ? 1 : S <= 1024 ? 2 : S <= 2048 ? 3 : S <= 4096 ? 4 : 5;
-            if (g == 5 || lds_fw64(m.h[i]) > (size_t)c.prop.sharedMemPerBlock) {
-                fprintf(stderr, "argweaver_amd: %d states per block exceed the FP64 kernel\n", S);
-                return 1;
-            }
-            lists[g].push_back(i);
-        }
-        std::vector<int> flat;
-        for (int g = 0; g < 5; g++) flat.insert(flat.end(), lists[g].begin(), lists[g].end());
-        HIP_OK(hipMemcpyAsync(m.d_list, flat.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-        int at = 0;
-        for (int g = 0; g < 5; g++) {
-            const int cnt = (int)lists[g].size();
-            if (!cnt) continue;
-            size_t sh = 0;
-            for (int i : lists[g]) sh = std::max(sh, lds_fw64(m.h[i]));
-            int *lp = m.d_list + at;
-#define FW64(NT, R)                                                                          \
-    do {                                                                                     \
-        auto kern = k_forward64_b<NT, R>;                                                    \
-        HIP_OK(hipFuncSetAttribute((const void *)kern,                                       \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));    \
-        hipLaunchKernelGGL(kern, dim3(cnt), dim3(NT), sh, s, m.dD, lp);                      \
-    } while (0)
-            if (g == 0) FW64(256, 1);
-            else if (g == 1) FW64(256, 2);
-            else if (g == 2) FW64(256, 4);
-            else if (g == 3) FW64(256, 8);
-            else FW64(256, 16);
-#undef FW64
-            HIP_OK(hipGetLastError());
-            at += cnt;
-        }
-        (void)hipEventRecord(c.ev[4], s);
-        hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD);
-        if (opt.traceback) {
-            at = 0;
-            for (int g = 0; g < 5; g++) {
-                const int cnt = (int)lists[g].size();
-                if (!cnt) continue;
-                size_t sh = 0;
-                for (int i : lists[g]) sh = std::max(sh, lds_tb(m.h[i]));
-                int *lp = m.d_list + at;
-#define TB64(RT)                                                                             \
-    do {                                                                                     \
-        auto kern = k_traceback64_b<RT>;                                                     \
-        HIP_OK(hipFuncSetAttribute((const void *)kern,                                       \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));    \
-        hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, s, m.dD, lp);                      \
-    } while (0)
-                if (g == 0) TB64(4);
-                else if (g == 1) TB64(8);
-                else if (g == 2) TB64(16);
-                else if (g == 3) TB64(32);
-                else TB64(64);
-#undef TB64
-                HIP_OK(hipGetLastError());
-                at += cnt;
-            }
-        }
-        (void)hipEventRecord(c.ev[5], s);
-    } else {
-        // FP32 column tiles
-        int ovf = 0;
-        HIP_OK(hipMemcpyAsync(&ovf, m.d_ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (opt.traceback) {
+        // tiles are grouped by R; launches of one pass run per group
+        auto tb = [&](int mode, const std::vector<Chunk> &list, const int *snap, int *flags) -> int {
+            int cnt[NRV] = {0}, off[NRV] = {0};
+            for (const Chunk &k : list) cnt[r_index(m.h[k.inst].R)]++;
+            for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
+            return launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
+                const int *sp = snap ? snap + o : nullptr;
+                int *fp = flags ? flags + o : nullptr;
+                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st)
+                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st);
+            });
+        };
         const int nch = (int)m.chunks.size();
         HIP_OK(hipMemcpyAsync(m.d_chunks, m.chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (ovf) {
-            fprintf(stderr, "argweaver_amd: a block needs %d slots per lane\n", ovf);
-            return 1;
-        }
-        int RM = 8, RT = 4, sstride = 1;
-        size_t shf = 0, sht = 0;
-        for (int i = 0; i < n; i++) {
-            RM = std::max(RM, m.h[i].RM);
-            sstride = std::max(sstride, m.h[i].Smax);
-            RT = std::max(RT, (m.h[i].Smax + 63) / 64);
-            shf = std::max(shf, lds_fw32(m.h[i]));
-            sht = std::max(sht, lds_tb(m.h[i]));
-        }
-        auto fw = [&](int mode, const Chunk *chp, int cnt, const float *snap, int *flags) -> int {
-#define FW32(R)                                                                              \
-    do {                                                                                     \
-        auto kern = k_forward32<R>;                                                          \
-        HIP_OK(hipFuncSetAttribute((const void *)kern,                                       \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shf));   \
-        hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), shf, s, m.dD, chp, mode, snap, sstride, flags, \
-                           opt.tol_rel, opt.tol_abs);                                  \
-    } while (0)
-            if (RM <= 8) FW32(8);
-            else if (RM <= 16) FW32(16);
-            else if (RM <= 32) FW32(32);
-            else FW32(64);
-#undef FW32
-            HIP_OK(hipGetLastError());
-            return 0;
-        };
-        auto tb = [&](int mode, const Chunk *chp, int cnt, const int *snap, int *flags) -> int {
-#define TB32(R)                                                                              \
-    do {                                                                                     \
-        auto kern = k_traceback32<R>;                                                        \
-        HIP_OK(hipFuncSetAttribute((const void *)kern,                                       \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sht));   \
-        hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sht, s, m.dD, chp, mode, snap, flags); \
-    } while (0)
-            if (RT <= 4) TB32(4);
-            else if (RT <= 8) TB32(8);
-            else if (RT <= 16) TB32(16);
-            else if (RT <= 32) TB32(32);
-            else TB32(64);
-#undef TB32
-            HIP_OK(hipGetLastError());
-            return 0;
-        };
-        // forward: speculative pass over every tile, then fix-up passes
-        if (fw(0, m.d_chunks, nch, nullptr, nullptr)) return 1;
+        if (tb(0, m.chunks, nullptr, nullptr)) return 1;
+        // fix-up passes: a tile re-sampled from its true boundary state
+        // until it coalesces with the stored path; a tile that does not
+        // coalesce passes the re-run on to the tile before it
         std::vector<int> redo;
         for (int j = 0; j < nch; j++)
-            if (!m.chunks[j].first) redo.push_back(j);
+            if (!m.chunks[j].last) redo.push_back(j);
+        if (!opt.fixup) redo.clear();
         std::vector<Chunk> sub;
         std::vector<int> flags;
-        Chunk *d_sub = m.d_chunks;   // re-used region after the full list
-        (void)d_sub;
-        if (!opt.fixup) redo.clear();
         while (!redo.empty()) {
-            rs.fw_passes++;
+            rs.tb_passes++;
             sub.clear();
             for (int j : redo) sub.push_back(m.chunks[j]);
             const int ns = (int)sub.size();
-            // sub-list after the full list inside the chunk buffer is not
-            // available; reuse the front of the buffer and restore later
             HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_fw_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_fsnap, sstride);
-            if (fw(1, m.d_chunks, ns, m.d_fsnap, m.d_flags)) return 1;
+            hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
+            if (tb(1, sub, m.d_tsnap, m.d_flags)) return 1;
             flags.resize(ns);
             HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
@@ -1746,45 +1587,13 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             for (int x = 0; x < ns; x++) {
                 const int j = redo[x];
                 const Chunk &k = m.chunks[j];
-                rs.fw_rerun_cols += (flags[x] < 0 ? k.c1 : flags[x] + 1) - k.c0;
-                if (flags[x] < 0 && !k.last) next.push_back(j + 1);
+                rs.tb_rerun_cols += k.c1 - (flags[x] < 0 ? k.c0 : flags[x]);
+                if (flags[x] < 0 && !k.first) next.push_back(j - 1);
             }
             redo.swap(next);
         }
-        HIP_OK(hipMemcpyAsync(m.d_chunks, m.chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
-        (void)hipEventRecord(c.ev[4], s);
-        hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD);
-        if (opt.traceback) {
-            if (tb(0, m.d_chunks, nch, nullptr, nullptr)) return 1;
-            redo.clear();
-            for (int j = 0; j < nch; j++)
-                if (!m.chunks[j].last) redo.push_back(j);
-            if (!opt.fixup) redo.clear();
-            while (!redo.empty()) {
-                rs.tb_passes++;
-                sub.clear();
-                for (int j : redo) sub.push_back(m.chunks[j]);
-                const int ns = (int)sub.size();
-                HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
-                hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
-                if (tb(1, m.d_chunks, ns, m.d_tsnap, m.d_flags)) return 1;
-                flags.resize(ns);
-                HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-                HIP_OK(hipStreamSynchronize(s));
-                std::vector<int> next;
-                for (int x = 0; x < ns; x++) {
-                    const int j = redo[x];
-                    const Chunk &k = m.chunks[j];
-                    rs.tb_rerun_cols += k.c1 - (flags[x] < 0 ? k.c0 : flags[x]);
-                    if (flags[x] < 0 && !k.first) next.push_back(j - 1);
-                }
-                redo.swap(next);
-            }
-            HIP_OK(hipMemcpyAsync(m.d_chunks, m.chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
-        }
-        (void)hipEventRecord(c.ev[5], s);
     }
-    std::vector<double> lnl(n);
+    (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -1817,17 +1626,34 @@ int Batch::copy_norms(int i, double *norms) const
     return 0;
 }
 
+// block-major view (block b holds blocklen_b rows of S_b states, the
+// reference's table layout) of the lane-run table
 int Batch::copy_alpha(int i, double *alpha) const
 {
     const Impl &m = *impl_;
     const DevInst &d = m.h[i];
+    const FlatTrees &ft = m.hosts[i].ft;
+    const int R = d.R, RW = 64 * R;
+    const size_t cells = (size_t)d.L * RW;
+    std::vector<double> tab(cells);
     if (m.prec_done == Precision::FP64) {
-        HIP_OK(hipMemcpy(alpha, d.alpha, sizeof(double) * d.W, hipMemcpyDeviceToHost));
-        return 0;
+        HIP_OK(hipMemcpy(tab.data(), d.alpha, sizeof(double) * cells, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t32(cells);
+        HIP_OK(hipMemcpy(t32.data(), d.alpha, sizeof(float) * cells, hipMemcpyDeviceToHost));
+        for (size_t x = 0; x < cells; x++) tab[x] = t32[x];
     }
-    std::vector<float> tmp(d.W);
-    HIP_OK(hipMemcpy(tmp.data(), d.alpha32, sizeof(float) * d.W, hipMemcpyDeviceToHost));
-    for (long long x = 0; x < d.W; x++) alpha[x] = tmp[x];
+    std::vector<int> pos;
+    long long o = 0;
+    for (int b = 0; b < ft.nblocks; b++) {
+        const int S = m.Sv[i][b];
+        pos.resize(S);
+        for (int k = 0; k < S; k++) pos[k] = lane_pos(k, R);
+        for (int cc = 0; cc < ft.len[b]; cc++) {
+            const double *row = &tab[(size_t)(ft.start[b] + cc) * RW];
+            for (int k = 0; k < S; k++) alpha[o++] = row[pos[k]];
+        }
+    }
     return 0;
 }
 
@@ -1874,10 +1700,9 @@ int device_info(char *buf, int n)
     return 0;
 }
 
-// one instance = a batch of one; the batch object is kept between calls so
-// device buffers are reused
+// one instance = a batch of one
 int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha, double *lnl,
-                 RunStats *stats, Precision prec, const double *prior, int chunk_cols)
+                 RunStats *stats, Precision prec, const double *prior, int tile_cols)
 {
     Batch bt;
     bt.add(hi, u, prior);
@@ -1885,7 +1710,7 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
     BatchOptions opt;
     opt.prec = prec;
     opt.traceback = (u != nullptr);
-    opt.chunk_cols = chunk_cols;
+    opt.tile_cols = tile_cols;
     if (bt.run(opt, stats)) return 1;
     if (path && u && bt.copy_path(0, path)) return 1;
     if (alpha && bt.copy_alpha(0, alpha)) return 1;

@@ -100,6 +100,36 @@ def cpu_baseline(A, insts, model, budget_s=15.0):
                    "; oracle forward + traceback"))
 
 
+def plain_fp64(A, insts, model, budget_s, log):
+    """The drop-in path: one instance per launch in FP64 (what one
+    arghmm_sample_thread call runs on the GPU), inputs resident, timed
+    instance after instance on a bounded sample of the workload."""
+    times, pops, rho, mu = model
+    rng = np.random.RandomState(7)
+    idx = rng.permutation(len(insts))
+    W = 0
+    t_used = 0.0
+    n = 0
+    for i in idx:
+        it = insts[i]
+        b = A.Batch()
+        b.add(it["trees"], it["seqs"], times, pops, rho, mu, it["k"] - 1, it["u"])
+        b.upload()
+        b.run(0, 0)   # warm (allocations)
+        t0 = time.perf_counter()
+        b.run(0, 0)
+        dt = time.perf_counter() - t0
+        W += b.W(0)
+        t_used += dt
+        n += 1
+        del b
+        if t_used >= budget_s:
+            break
+    log("plain fp64: %d instances one per launch, %.3g column-states in %.2f s" % (n, W, t_used))
+    return dict(value=W / t_used, unit="column-states/s", instances=n, column_states=W,
+                seconds=t_used)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,14 +137,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--chunk", type=int, default=5000, help="FP32 column tile length")
-    ap.add_argument("--tol-rel", type=float, default=1e-4,
-                    help="fix-up tolerance of a re-run FP32 tile column (relative)")
-    ap.add_argument("--tol-abs", type=float, default=1e-7,
-                    help="fix-up tolerance of a re-run FP32 tile column (absolute)")
+    ap.add_argument("--tile", type=int, default=2000, help="traceback tile length (columns)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--plain-budget", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the threading instances shard across GPUs with no "
@@ -137,41 +164,48 @@ def main():
     Wtot = sum(batch.W(i) for i in range(len(insts)))
     log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
 
-    def timed(prec, chunk):
-        kw = dict(tol_rel=args.tol_rel, tol_abs=args.tol_abs)
+    def timed(prec):
         for _ in range(args.warmup):
-            batch.run(prec, chunk, **kw)
+            batch.run(prec, args.tile)
         t0 = time.perf_counter()
-        stats = [batch.run(prec, chunk, **kw) for _ in range(args.steps)]
+        stats = [batch.run(prec, args.tile) for _ in range(args.steps)]
         wall = (time.perf_counter() - t0) / args.steps
         return wall, stats
 
-    wall32, st32 = timed(1, args.chunk)
-    fwd_ms = float(np.mean([s["ms_forward"] for s in st32]))
-    tb_ms = float(np.mean([s["ms_traceback"] for s in st32]))
-    log("fp32 tiles: %.1f ms/step (forward %.1f, traceback %.1f, passes fw %d tb %d)"
-        % (wall32 * 1e3, fwd_ms, tb_ms, st32[-1]["fw_passes"], st32[-1]["tb_passes"]))
+    def mean(st, k):
+        return float(np.mean([x[k] for x in st]))
+
+    wall32, st32 = timed(1)
+    log("batched fp32: %.1f ms/step (prep %.1f, emit %.1f, forward %.1f, traceback %.1f, "
+        "tb passes %d)" % (wall32 * 1e3, mean(st32, "ms_prep"), mean(st32, "ms_emit"),
+                           mean(st32, "ms_forward"), mean(st32, "ms_traceback"),
+                           st32[-1]["tb_passes"]))
     lnl32 = [batch.lnl(i) for i in range(len(insts))]
 
-    extra = {}
+    extra = {"stages_ms": {k: mean(st32, k) for k in ("ms_prep", "ms_emit", "ms_forward",
+                                                      "ms_traceback", "ms_total")}}
     if not args.no_fp64:
-        wall64, st64 = timed(0, 0)
+        wall64, st64 = timed(0)
         lnl64 = [batch.lnl(i) for i in range(len(insts))]
         rel = max(abs(a - b) / abs(b) for a, b in zip(lnl32, lnl64))
-        extra["fp64_plain"] = dict(value=Wtot / wall64, ms_per_step=wall64 * 1e3,
-                                   ms_forward=float(np.mean([s["ms_forward"] for s in st64])),
-                                   ms_traceback=float(np.mean([s["ms_traceback"] for s in st64])))
-        extra["fp32_vs_fp64"] = (Wtot / wall32) / (Wtot / wall64)
-        extra["max_rel_dlnl_fp32_vs_fp64"] = rel
-        log("fp64 plain: %.1f ms/step, max |dlnl|/|lnl| fp32 vs fp64 %.2e" % (wall64 * 1e3, rel))
+        log("batched fp64: %.1f ms/step; max |dlogL|/|logL| fp32 vs fp64 %.2e"
+            % (wall64 * 1e3, rel))
+        plain = plain_fp64(A, insts, model, args.plain_budget, log)
+        extra["fp64_batched"] = dict(value=Wtot / wall64, ms_per_step=wall64 * 1e3,
+                                     ms_forward=mean(st64, "ms_forward"),
+                                     ms_traceback=mean(st64, "ms_traceback"))
+        extra["fp64_plain"] = plain
+        extra["fp32_batched_vs_fp64_plain"] = (Wtot / wall32) / plain["value"]
+        extra["max_rel_dlogL_fp32_vs_fp64"] = rel
 
-    # dominant kernel: the FP32 forward; algorithmic bytes = one FP32 write
-    # per column-state (the traceback reads it back)
+    # dominant kernel: the FP32 forward (k_fw_lane<float>), one launch per
+    # step; algorithmic bytes = one FP32 table entry written per column-state
+    fwd_ms = mean(st32, "ms_forward")
     achieved = (Wtot * 4) / (fwd_ms * 1e-3)
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK, traffic=None,
-                    kernel="k_forward32", algorithmic_bytes_per_launch=Wtot * 4,
-                    ms_per_launch_sum=fwd_ms)
+                    kernel="k_fw_lane<float>", algorithmic_bytes_per_launch=Wtot * 4,
+                    ms_per_launch=fwd_ms)
 
     cpu = None
     if not args.no_cpu:
@@ -193,7 +227,7 @@ def main():
         "data": "synthetic (seeded DSMC ARGs, one leaf removed per instance)",
         "config": {"workload": "batched: %d instances, k~U{8..64}, L log-uniform 50kb-2Mb at c=10, "
                                "ntimes=20" % len(insts),
-                   "instances": len(insts), "column_states": Wtot, "chunk_cols": args.chunk},
+                   "instances": len(insts), "column_states": Wtot, "tile_cols": args.tile},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

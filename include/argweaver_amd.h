@@ -147,21 +147,24 @@ LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int
  * instance (the ARG is flattened at once; sequence rows and uniforms must
  * stay valid until awamd_batch_upload, which copies every input to HBM).
  * awamd_batch_run executes classification, block preparation, emissions,
- * forward and traceback for all instances.  precision 0: plain FP64 path,
- * one workgroup per instance.  precision 1: FP32 column tiles of
- * chunk_cols columns (0: one tile per instance) with fix-up passes.
- * stats[11]: prep, emit, forward, traceback and total ms, W, variant
- * columns, forward/traceback fix-up passes and re-run columns. */
+ * forward and traceback for all instances.  precision 0: FP64 forward table
+ * (the drop-in arithmetic); precision 1: FP32 table and recursion with
+ * per-column rescaling, FP64 normalisers, log-likelihood and traceback CDF.
+ * The forward runs one wave per instance; the traceback runs in tiles of
+ * tile_cols columns (0: one tile per instance) with fix-up passes.
+ * stats[9]: prep, emit, forward, traceback and total ms, W, variant columns,
+ * traceback fix-up passes and re-sampled columns. */
 void *awamd_batch_new(void);
 void awamd_batch_free(void *batch);
 int awamd_batch_add(void *batch, LocalTrees *trees, const double *times, int ntimes,
                     const double *popsizes, double rho, double mu, char **seqs, int new_chrom,
                     const double *uniforms);
 int awamd_batch_upload(void *batch);
-int awamd_batch_run(void *batch, int precision, int chunk_cols, int traceback, double *stats);
-/* as awamd_batch_run, with the FP32 tile fix-up switch and agreement tolerance */
-int awamd_batch_run_opts(void *batch, int precision, int chunk_cols, int traceback, int fixup,
-                         double tol_rel, double tol_abs, double *stats);
+int awamd_batch_run(void *batch, int precision, int tile_cols, int traceback, double *stats);
+/* as awamd_batch_run, with the traceback fix-up switch (0: speculative tiles
+ * only, for diagnostics) */
+int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int traceback, int fixup,
+                         double *stats);
 int awamd_batch_path(void *batch, int i, int *path);
 int awamd_batch_alpha(void *batch, int i, double *alpha);
 int awamd_batch_norms(void *batch, int i, double *norms);
