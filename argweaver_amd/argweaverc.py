@@ -124,11 +124,22 @@ awamd_batch_upload = _export("awamd_batch_upload", c_int, [c_void_p])
 awamd_batch_run = _export("awamd_batch_run", c_int, [c_void_p, c_int, c_int, c_int, c_double_p])
 awamd_batch_run_opts = _export(
     "awamd_batch_run_opts", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p])
+awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
 awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
 awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_lnl = _export("awamd_batch_lnl", c_double, [c_void_p, c_int])
 awamd_batch_W = _export("awamd_batch_W", c_longlong, [c_void_p, c_int])
+
+awamd_server_new = _export("awamd_server_new", c_void_p, [c_int, c_double, c_int, c_int, c_int])
+awamd_server_free = _export("awamd_server_free", None, [c_void_p])
+awamd_server_submit = _export(
+    "awamd_server_submit", c_int,
+    [c_void_p, c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p, c_int,
+     c_double_p])
+awamd_server_wait = _export("awamd_server_wait", c_int,
+                            [c_void_p, c_int, c_int_p, c_double_p, c_double_p])
+awamd_server_now = _export("awamd_server_now", c_double, [c_void_p])
 
 _libc = ctypes.CDLL(None)
 
@@ -399,6 +410,16 @@ class Batch(object):
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
 
+    def tune(self, precision=1, candidates=(0, 500, 1000, 2000, 4000, 8000)):
+        """Autotune the traceback tile length on this batch; returns
+        (best, {candidate: ms})."""
+        cand = np.ascontiguousarray(candidates, np.int32)
+        ms = np.zeros(len(cand), np.float64)
+        best = awamd_batch_tune(self.ptr, precision, _iptr(cand), len(cand), _dptr(ms))
+        if best < 0:
+            raise RuntimeError("awamd_batch_tune failed")
+        return int(best), dict(zip(cand.tolist(), ms.tolist()))
+
     def path(self, i):
         out = np.zeros(self.L[i], np.int32)
         awamd_batch_path(self.ptr, i, _iptr(out))
@@ -424,3 +445,45 @@ class Batch(object):
         if getattr(self, "ptr", None):
             awamd_batch_free(self.ptr)
             self.ptr = None
+
+
+class Server(object):
+    """Continuous-batching server (awamd_server_*): submit() from any thread,
+    wait() for a result; requests are admitted by device-memory budget and
+    overlap on the GPU with the batches already running."""
+
+    def __init__(self, workers=4, hbm_budget_gb=64.0, precision=1, tile_cols=2000, max_batch=64):
+        self.ptr = awamd_server_new(workers, hbm_budget_gb, precision, tile_cols, max_batch)
+        if not self.ptr:
+            raise RuntimeError("awamd_server_new failed")
+        self.L = {}
+
+    def submit(self, trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms):
+        times = np.ascontiguousarray(times, np.float64)
+        popsizes = np.ascontiguousarray(popsizes, np.float64)
+        cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        i = awamd_server_submit(self.ptr, trees, _dptr(times), len(times), _dptr(popsizes), rho, mu,
+                                cs.ptr, new_chrom, _dptr(u))
+        self.L[i] = cs.seqlen
+        return i
+
+    def wait(self, i):
+        path = np.zeros(self.L.pop(i), np.int32)
+        lnl = c_double(0)
+        t = np.zeros(3, np.float64)
+        rc = awamd_server_wait(self.ptr, i, _iptr(path), ctypes.byref(lnl), _dptr(t))
+        if rc != 0:
+            raise RuntimeError("request %d failed (%d)" % (i, rc))
+        return dict(path=path, lnl=lnl.value, t_submit=t[0], t_start=t[1], t_done=t[2])
+
+    def now(self):
+        return awamd_server_now(self.ptr)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            awamd_server_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()

@@ -12,6 +12,7 @@
 
 #include "../../include/argweaver_amd.h"
 #include "engine.h"
+#include "serve.h"
 #include "local_trees.h"
 
 namespace aw {
@@ -624,6 +625,12 @@ int awamd_batch_run(void *b, int precision, int tile_cols, int traceback, double
     return awamd_batch_run_opts(b, precision, tile_cols, traceback, 1, stats);
 }
 
+int awamd_batch_tune(void *b, int precision, const int *cand, int ncand, double *ms_out)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.tune_tiles(
+        precision == 1 ? Precision::FP32 : Precision::FP64, cand, ncand, ms_out);
+}
+
 int awamd_batch_path(void *b, int i, int *path)
 {
     return reinterpret_cast<AwBatch *>(b)->batch.copy_path(i, path);
@@ -686,3 +693,54 @@ extern "C" LocalTrees *awamd_simulate(int k, int seqlen, const double *times, in
     return reinterpret_cast<LocalTrees *>(
         aw::simulate_arg(k, seqlen, times, ntimes, popsizes, rho, mu, seed, seqs));
 }
+
+//---------------------------------------------------------------------------
+// continuous-batching server
+
+extern "C" {
+
+void *awamd_server_new(int workers, double hbm_budget_gb, int precision, int tile_cols,
+                       int max_batch)
+{
+    if (!device_available()) {
+        fprintf(stderr, "argweaver_amd: awamd_server_new needs a HIP device\n");
+        return nullptr;
+    }
+    ServeOptions o;
+    o.workers = workers;
+    o.hbm_budget = (size_t)(hbm_budget_gb * 1073741824.0);
+    o.max_batch = max_batch;
+    o.run.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
+    o.run.tile_cols = tile_cols;
+    return new Server(o);
+}
+
+void awamd_server_free(void *server) { delete reinterpret_cast<Server *>(server); }
+
+int awamd_server_submit(void *server, LocalTrees *trees, const double *times, int ntimes,
+                        const double *popsizes, double rho, double mu, char **seqs,
+                        int new_chrom, const double *uniforms)
+{
+    HostInstance hi;
+    build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu, seqs,
+                   new_chrom, hi);
+    return reinterpret_cast<Server *>(server)->submit(hi, uniforms);
+}
+
+int awamd_server_wait(void *server, int id, int *path, double *lnl, double *times3)
+{
+    ServeResult r;
+    if (reinterpret_cast<Server *>(server)->wait(id, &r)) return -1;
+    if (path) std::copy(r.path.begin(), r.path.end(), path);
+    if (lnl) *lnl = r.lnl;
+    if (times3) {
+        times3[0] = r.t_submit;
+        times3[1] = r.t_start;
+        times3[2] = r.t_done;
+    }
+    return r.status;
+}
+
+double awamd_server_now(void *server) { return reinterpret_cast<Server *>(server)->now(); }
+
+}  // extern "C"

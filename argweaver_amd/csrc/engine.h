@@ -129,8 +129,16 @@ public:
     Batch &operator=(const Batch &) = delete;
     // the instance's row pointers and uniforms must stay valid until upload()
     int add(const HostInstance &hi, const double *u, const double *prior = nullptr);
+    // drop the instances, keep the device buffers for the next set
+    void clear();
     int upload();
     int run(const BatchOptions &opt, RunStats *st = nullptr);
+    // Autotuner of the traceback schedule: runs the forward once, then times
+    // the traceback alone for each candidate tile length (0: one tile per
+    // instance) on the resident tables; returns the fastest (ms per
+    // candidate in ms_out when given), -1 on error.  Leaves the batch with
+    // the path of the last candidate.
+    int tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out = nullptr);
     int copy_path(int i, int *path) const;
     int copy_alpha(int i, double *alpha) const;   // block-major FP64 view of the table
     int copy_norms(int i, double *norms) const;

@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1091,14 +1092,11 @@ namespace {
 constexpr int NRV = 7;
 constexpr int RVALS[NRV] = {1, 2, 4, 6, 8, 12, 16};
 
+// Device-wide state, initialised once per process.
 struct Ctx {
     bool init = false;
     int dev = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t side[NRV];        // one per R group, all at the default priority
-    hipEvent_t fork, join[NRV];
     hipDeviceProp_t prop{};
-    hipEvent_t ev[8];
 };
 
 Ctx &ctx()
@@ -1109,18 +1107,49 @@ Ctx &ctx()
 
 int ctx_init()
 {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
     Ctx &c = ctx();
     if (c.init) return 0;
     HIP_OK(hipGetDevice(&c.dev));
     HIP_OK(hipGetDeviceProperties(&c.prop, c.dev));
-    HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    for (auto &st : c.side) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    HIP_OK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
-    for (auto &e : c.join) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &e : c.ev) HIP_OK(hipEventCreate(&e));
     c.init = true;
     return 0;
 }
+
+// Streams and events of one batch: the main stream plus one side stream per
+// R group, all at the default priority.  Each batch owns its own, so batches
+// driven from different host threads overlap on the device.
+struct Streams {
+    bool init = false;
+    hipStream_t main = nullptr;
+    hipStream_t side[NRV];
+    hipEvent_t fork, join[NRV];
+    hipEvent_t ev[8];
+    int create()
+    {
+        if (init) return 0;
+        HIP_OK(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+        for (auto &st : side) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        for (auto &e : join) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+        init = true;
+        return 0;
+    }
+    void destroy()
+    {
+        if (!init) return;
+        (void)hipStreamSynchronize(main);
+        for (auto &st : side) (void)hipStreamSynchronize(st);
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : join) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(fork);
+        for (auto &st : side) (void)hipStreamDestroy(st);
+        (void)hipStreamDestroy(main);
+        init = false;
+    }
+};
 
 // grow-only device buffer
 struct DevBuf {
@@ -1175,15 +1204,15 @@ int r_index(int R)
 // Launch per-R work on the side streams, concurrently, joined back into the
 // main stream.  cnt[g] items of group g start at offset off[g].
 template <class F>
-int launch_groups(Ctx &c, const int *cnt, const int *off, F fn)
+int launch_groups(Streams &c, const int *cnt, const int *off, F fn)
 {
-    HIP_OK(hipEventRecord(c.fork, c.stream));
+    HIP_OK(hipEventRecord(c.fork, c.main));
     for (int g = 0; g < NRV; g++) {
         if (!cnt[g]) continue;
         HIP_OK(hipStreamWaitEvent(c.side[g], c.fork, 0));
         if (fn(g, off[g], cnt[g], c.side[g])) return 1;
         HIP_OK(hipEventRecord(c.join[g], c.side[g]));
-        HIP_OK(hipStreamWaitEvent(c.stream, c.join[g], 0));
+        HIP_OK(hipStreamWaitEvent(c.main, c.join[g], 0));
     }
     return 0;
 }
@@ -1251,7 +1280,8 @@ struct Batch::Impl {
     std::vector<double> lnl;
     std::vector<int> nvar;
     std::vector<Chunk> chunks;
-    DevBuf in, work, tab, aux, vbuf;
+    DevBuf in, work, tab, aux, vbuf, tiles;
+    Streams st;
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
     int *d_blkoff = nullptr, *d_voff = nullptr, *d_nvar = nullptr, *d_list = nullptr;
@@ -1262,17 +1292,35 @@ struct Batch::Impl {
 
     ~Impl()
     {
+        st.destroy();
         in.release();
         work.release();
         tab.release();
         aux.release();
         vbuf.release();
+        tiles.release();
     }
 };
 
 Batch::Batch() : impl_(new Impl) {}
 Batch::~Batch() { delete impl_; }
 int Batch::size() const { return (int)impl_->hosts.size(); }
+
+void Batch::clear()
+{
+    Impl &m = *impl_;
+    m.hosts.clear();
+    m.us.clear();
+    m.priors.clear();
+    m.Sv.clear();
+    m.soff.clear();
+    m.h.clear();
+    m.W.clear();
+    m.lnl.clear();
+    m.nvar.clear();
+    m.chunks.clear();
+    m.uploaded = false;
+}
 double Batch::lnl(int i) const { return impl_->lnl[i]; }
 long long Batch::W(int i) const { return impl_->W[i]; }
 
@@ -1296,8 +1344,9 @@ int Batch::add(const HostInstance &hi, const double *u, const double *prior)
 int Batch::upload()
 {
     if (ctx_init()) return 1;
-    Ctx &c = ctx();
     Impl &m = *impl_;
+    if (m.st.create()) return 1;
+    Streams &c = m.st;
     const int n = (int)m.hosts.size();
     m.h.assign(n, DevInst{});
     m.soff.assign(n, {});
@@ -1346,7 +1395,7 @@ int Batch::upload()
             d.seq = p.take<char>((size_t)d.nrows * L);
             d.u = p.take<double>(L);
             if (pass == 1) {
-                hipStream_t s = c.stream;
+                hipStream_t s = c.main;
                 HIP_OK(hipMemcpyAsync((void *)d.model, &hi.model, sizeof(Model), hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync((void *)d.parent, ft.parent.data(), sizeof(int) * B * N, hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync((void *)d.c0, ft.child0.data(), sizeof(int) * B * N, hipMemcpyHostToDevice, s));
@@ -1365,21 +1414,135 @@ int Batch::upload()
         }
         if (pass == 0 && m.in.ensure(p.used + 4096)) return 1;
     }
-    HIP_OK(hipStreamSynchronize(c.stream));
+    HIP_OK(hipStreamSynchronize(c.main));
     // host rows may be released now
     for (auto &hi : m.hosts) hi.rows.clear();
     m.uploaded = true;
     return 0;
 }
 
+// The traceback of every instance whose forward table is resident: tiles of
+// tile_cols columns (0: one per instance), grouped by R, then fix-up passes
+// in which a tile is re-sampled from its true boundary state until it
+// coalesces with the stored path; a tile that does not coalesce passes the
+// re-run on to the tile before it.
+static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, bool fixup, RunStats &rs)
+{
+    Streams &c = m.st;
+    hipStream_t s = c.main;
+    const int n = (int)m.h.size();
+    const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
+    std::vector<Chunk> &chs = m.chunks;
+    chs.clear();
+    std::vector<int> byr(n);
+    std::iota(byr.begin(), byr.end(), 0);
+    std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return m.h[a].R < m.h[b].R; });
+    size_t sht = 0;
+    for (int i : byr) {
+        const DevInst &d = m.h[i];
+        const int tile = tile_cols > 0 ? tile_cols : d.L;
+        const int nt = (d.L + tile - 1) / tile;
+        for (int t = 0; t < nt; t++) {
+            Chunk k;
+            k.inst = i;
+            k.c0 = t * tile;
+            k.c1 = std::min(d.L, (t + 1) * tile);
+            k.first = (t == 0);
+            k.last = (t == nt - 1);
+            chs.push_back(k);
+        }
+        // (a,b) and (c,a,b) tables, then two row batches
+        const int T1 = d.T - 1;
+        const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
+        sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
+    }
+    const int nch = (int)chs.size();
+    for (int pass = 0; pass < 2; pass++) {
+        Plan p;
+        p.base = pass ? m.tiles.p : nullptr;
+        m.d_flags = p.take<int>(nch);
+        m.d_tsnap = p.take<int>(nch);
+        m.d_chunks = p.take<Chunk>(nch);
+        if (pass == 0 && m.tiles.ensure(p.used + 4096)) return 1;
+    }
+    auto tb = [&](int mode, const std::vector<Chunk> &list, const int *snap, int *flags) -> int {
+        int cnt[NRV] = {0}, off[NRV] = {0};
+        for (const Chunk &k : list) cnt[r_index(m.h[k.inst].R)]++;
+        for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
+        return launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
+            const int *sp = snap ? snap + o : nullptr;
+            int *fp = flags ? flags + o : nullptr;
+            return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st)
+                        : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st);
+        });
+    };
+    HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
+    if (tb(0, chs, nullptr, nullptr)) return 1;
+    std::vector<int> redo;
+    for (int j = 0; j < nch; j++)
+        if (!chs[j].last) redo.push_back(j);
+    if (!fixup) redo.clear();
+    std::vector<Chunk> sub;
+    std::vector<int> flags;
+    while (!redo.empty()) {
+        rs.tb_passes++;
+        sub.clear();
+        for (int j : redo) sub.push_back(chs[j]);
+        const int ns = (int)sub.size();
+        HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
+        if (tb(1, sub, m.d_tsnap, m.d_flags)) return 1;
+        flags.resize(ns);
+        HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        std::vector<int> next;
+        for (int x = 0; x < ns; x++) {
+            const int j = redo[x];
+            const Chunk &k = chs[j];
+            rs.tb_rerun_cols += k.c1 - (flags[x] < 0 ? k.c0 : flags[x]);
+            if (flags[x] < 0 && !k.first) next.push_back(j - 1);
+        }
+        redo.swap(next);
+    }
+    return 0;
+}
+
+int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out)
+{
+    BatchOptions opt;
+    opt.prec = prec;
+    opt.traceback = false;
+    if (run(opt)) return -1;
+    Impl &m = *impl_;
+    Streams &c = m.st;
+    const bool fp32 = prec == Precision::FP32;
+    int best = cand[0];
+    float best_ms = 0;
+    for (int i = 0; i < ncand; i++) {
+        RunStats rs;
+        (void)hipEventRecord(c.ev[6], c.main);
+        if (traceback_stage(m, fp32, cand[i], true, rs)) return -1;
+        (void)hipEventRecord(c.ev[7], c.main);
+        HIP_OK(hipStreamSynchronize(c.main));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c.ev[6], c.ev[7]);
+        if (ms_out) ms_out[i] = ms;
+        if (i == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = cand[i];
+        }
+    }
+    return best;
+}
+
 int Batch::run(const BatchOptions &opt, RunStats *st)
 {
     if (ctx_init()) return 1;
-    Ctx &c = ctx();
     Impl &m = *impl_;
     if (!m.uploaded && upload()) return 1;
+    Streams &c = m.st;
     const int n = (int)m.hosts.size();
-    hipStream_t s = c.stream;
+    hipStream_t s = c.main;
     const bool fp32 = opt.prec == Precision::FP32;
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
 
@@ -1429,43 +1592,17 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         }
         if (pass == 0 && m.tab.ensure(p.used + 4096)) return 1;
     }
-    // traceback tiles and auxiliary arrays
-    {
-        std::vector<Chunk> &chs = m.chunks;
-        chs.clear();
-        std::vector<int> byr(n);
-        std::iota(byr.begin(), byr.end(), 0);
-        std::stable_sort(byr.begin(), byr.end(),
-                         [&](int a, int b) { return m.h[a].R < m.h[b].R; });
-        for (int i : byr) {
-            const DevInst &d = m.h[i];
-            const int tile = opt.tile_cols > 0 ? opt.tile_cols : d.L;
-            const int nt = (d.L + tile - 1) / tile;
-            for (int t = 0; t < nt; t++) {
-                Chunk k;
-                k.inst = i;
-                k.c0 = t * tile;
-                k.c1 = std::min(d.L, (t + 1) * tile);
-                k.first = (t == 0);
-                k.last = (t == nt - 1);
-                chs.push_back(k);
-            }
-        }
-        const size_t nch = chs.size();
-        for (int pass = 0; pass < 2; pass++) {
-            Plan p;
-            p.base = pass ? m.aux.p : nullptr;
-            m.dD = p.take<DevInst>(n);
-            m.d_coloff = p.take<long long>(n + 1);
-            m.d_blkoff = p.take<int>(n + 1);
-            m.d_voff = p.take<int>(n + 1);
-            m.d_nvar = p.take<int>(n);
-            m.d_list = p.take<int>(n);
-            m.d_flags = p.take<int>(nch);
-            m.d_tsnap = p.take<int>(nch);
-            m.d_chunks = p.take<Chunk>(nch);
-            if (pass == 0 && m.aux.ensure(p.used + 4096)) return 1;
-        }
+    // auxiliary arrays
+    for (int pass = 0; pass < 2; pass++) {
+        Plan p;
+        p.base = pass ? m.aux.p : nullptr;
+        m.dD = p.take<DevInst>(n);
+        m.d_coloff = p.take<long long>(n + 1);
+        m.d_blkoff = p.take<int>(n + 1);
+        m.d_voff = p.take<int>(n + 1);
+        m.d_nvar = p.take<int>(n);
+        m.d_list = p.take<int>(n);
+        if (pass == 0 && m.aux.ensure(p.used + 4096)) return 1;
     }
 
     (void)hipEventRecord(c.ev[0], s);
@@ -1524,7 +1661,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     for (int i = 0; i < n; i++) gcnt[r_index(m.h[i].R)]++;
     for (int g = 1; g < NRV; g++) goff[g] = goff[g - 1] + gcnt[g - 1];
     HIP_OK(hipMemcpyAsync(m.d_list, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    size_t shf[NRV] = {0}, sht = 0;
+    size_t shf[NRV] = {0};
     int t1max[NRV] = {0};
     for (int i = 0; i < n; i++) {
         const DevInst &d = m.h[i];
@@ -1533,9 +1670,6 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         size_t &x = shf[r_index(d.R)];
         const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
         x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R));
-        // (a,b) and (c,a,b) tables, then two row batches
-        const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
-        sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
     if (launch_groups(c, gcnt, goff, [&](int g, int off, int cnt, hipStream_t st) {
             return fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
@@ -1547,52 +1681,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipGetLastError());
 
     RunStats rs;
-    if (opt.traceback) {
-        // tiles are grouped by R; launches of one pass run per group
-        auto tb = [&](int mode, const std::vector<Chunk> &list, const int *snap, int *flags) -> int {
-            int cnt[NRV] = {0}, off[NRV] = {0};
-            for (const Chunk &k : list) cnt[r_index(m.h[k.inst].R)]++;
-            for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
-            return launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
-                const int *sp = snap ? snap + o : nullptr;
-                int *fp = flags ? flags + o : nullptr;
-                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st)
-                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st);
-            });
-        };
-        const int nch = (int)m.chunks.size();
-        HIP_OK(hipMemcpyAsync(m.d_chunks, m.chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
-        if (tb(0, m.chunks, nullptr, nullptr)) return 1;
-        // fix-up passes: a tile re-sampled from its true boundary state
-        // until it coalesces with the stored path; a tile that does not
-        // coalesce passes the re-run on to the tile before it
-        std::vector<int> redo;
-        for (int j = 0; j < nch; j++)
-            if (!m.chunks[j].last) redo.push_back(j);
-        if (!opt.fixup) redo.clear();
-        std::vector<Chunk> sub;
-        std::vector<int> flags;
-        while (!redo.empty()) {
-            rs.tb_passes++;
-            sub.clear();
-            for (int j : redo) sub.push_back(m.chunks[j]);
-            const int ns = (int)sub.size();
-            HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
-            if (tb(1, sub, m.d_tsnap, m.d_flags)) return 1;
-            flags.resize(ns);
-            HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-            HIP_OK(hipStreamSynchronize(s));
-            std::vector<int> next;
-            for (int x = 0; x < ns; x++) {
-                const int j = redo[x];
-                const Chunk &k = m.chunks[j];
-                rs.tb_rerun_cols += k.c1 - (flags[x] < 0 ? k.c0 : flags[x]);
-                if (flags[x] < 0 && !k.first) next.push_back(j - 1);
-            }
-            redo.swap(next);
-        }
-    }
+    if (opt.traceback && traceback_stage(m, fp32, opt.tile_cols, opt.fixup, rs)) return 1;
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -1,0 +1,83 @@
+// serve.h -- continuous batching of threading requests on one GPU.
+//
+// Requests (one threading instance each: the ARG with one lineage removed,
+// the sequences, the uniforms of the traceback) are submitted from any host
+// thread.  A pool of workers, each owning a reusable Batch with its own HIP
+// streams, admits queued requests in arrival order as long as the estimated
+// HBM footprint of everything in flight stays within the budget; a worker
+// that becomes free takes every request that fits, so requests arriving
+// while other batches run are admitted at once and their kernels overlap
+// with the running ones on the device.  Each request's result does not
+// depend on the batch it ran in.
+#pragma once
+
+#include <condition_variable>
+#include <cstddef>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+namespace aw {
+
+struct ServeOptions {
+    int workers = 4;                       // batches in flight
+    size_t hbm_budget = size_t(64) << 30;  // bytes of device memory for admitted work
+    int max_batch = 64;                    // requests per batch
+    BatchOptions run;                      // precision and traceback tiles
+};
+
+struct ServeResult {
+    int id = -1;
+    int status = 0;          // 0 ok, nonzero: the batch failed
+    double lnl = 0.0;
+    std::vector<int> path;
+    long long W = 0;         // column-states of the request
+    double t_submit = 0, t_start = 0, t_done = 0;   // seconds on the server clock
+    int batch_size = 0;
+};
+
+class Server {
+public:
+    explicit Server(const ServeOptions &opt);
+    ~Server();
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+
+    // Takes a copy of the sequence rows and the uniforms.  Returns the id.
+    int submit(const HostInstance &hi, const double *u);
+    // Blocks until request id is done; the result is removed from the server.
+    int wait(int id, ServeResult *out);
+    // Seconds since the server started (the clock of the result times).
+    double now() const;
+    // Device bytes a request needs while it runs (the admission estimate).
+    static size_t footprint(const HostInstance &hi, Precision prec);
+
+private:
+    struct Request {
+        int id;
+        HostInstance hi;
+        std::vector<std::string> rows;   // owned copies of the sequence rows
+        std::vector<double> u;
+        size_t bytes;
+        double t_submit;
+    };
+    void worker();
+
+    ServeOptions opt_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<Request *> queue_;
+    std::map<int, ServeResult> done_;
+    size_t in_use_ = 0;
+    int next_id_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> threads_;
+    double t0_;
+};
+
+}  // namespace aw

@@ -130,6 +130,45 @@ def plain_fp64(A, insts, model, budget_s, log):
                 seconds=t_used)
 
 
+def serving(A, insts, model, rate, nreq, seed, log, workers=4, budget_gb=96.0, tile=2000):
+    """Poisson arrivals at `rate` requests/s of instances drawn from the
+    workload, served with continuous batching (FP32); per-request latency =
+    finish - arrival on the server clock."""
+    import threading
+    rng = np.random.RandomState(seed)
+    picks = rng.randint(0, len(insts), size=nreq)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    srv = A.Server(workers=workers, hbm_budget_gb=budget_gb, precision=1, tile_cols=tile,
+                   max_batch=64)
+    ids = [None] * nreq
+
+    def producer():
+        t_next = srv.now()
+        for j, i in enumerate(picks):
+            t_next += gaps[j]
+            dt = t_next - srv.now()
+            if dt > 0:
+                time.sleep(dt)
+            it = insts[i]
+            ids[j] = srv.submit(it["trees"], it["seqs"], *model, it["k"] - 1, it["u"])
+
+    t0 = srv.now()
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    res = [srv.wait(i) for i in ids]
+    wall = srv.now() - t0
+    srv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
+    W = float(sum(insts[i]["W"] for i in picks))
+    out = dict(rate=rate, requests=nreq, p50_ms=float(np.percentile(lat, 50)),
+               p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
+               throughput=W / wall, unit="column-states/s", wall_s=wall, workers=workers)
+    log("serving %.0f req/s: p50 %.1f ms, p99 %.1f ms, %.3g column-states/s"
+        % (rate, out["p50_ms"], out["p99_ms"], out["throughput"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,11 +176,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--tile", type=int, default=2000, help="traceback tile length (columns)")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="traceback tile length in columns (-1: autotune, 0: one tile per instance)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--plain-budget", type=float, default=10.0)
+    ap.add_argument("--serve-rates", default="50,200", help="Poisson request rates (req/s)")
+    ap.add_argument("--serve-requests", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the threading instances shard across GPUs with no "
@@ -162,7 +204,14 @@ def main():
         batch.add(it["trees"], it["seqs"], *model, it["k"] - 1, it["u"])
     batch.upload()
     Wtot = sum(batch.W(i) for i in range(len(insts)))
+    for i, it in enumerate(insts):
+        it["W"] = batch.W(i)
     log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
+
+    if args.tile < 0:
+        tile, tms = batch.tune(1)
+        log("autotuned traceback tile: %d (%s)" % (tile, ", ".join("%d: %.1f ms" % kv for kv in tms.items())))
+        args.tile = tile
 
     def timed(prec):
         for _ in range(args.warmup):
@@ -197,6 +246,11 @@ def main():
         extra["fp64_plain"] = plain
         extra["fp32_batched_vs_fp64_plain"] = (Wtot / wall32) / plain["value"]
         extra["max_rel_dlogL_fp32_vs_fp64"] = rel
+
+    if args.serve_rates:
+        extra["serving"] = [serving(A, insts, model, float(r), args.serve_requests, args.seed + j,
+                                    log, tile=args.tile)
+                            for j, r in enumerate(args.serve_rates.split(","))]
 
     # dominant kernel: the FP32 forward (k_fw_lane<float>), one launch per
     # step; algorithmic bytes = one FP32 table entry written per column-state

@@ -165,6 +165,10 @@ int awamd_batch_run(void *batch, int precision, int tile_cols, int traceback, do
  * only, for diagnostics) */
 int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int traceback, int fixup,
                          double *stats);
+/* Traceback-schedule autotuner: runs the forward once, times the traceback
+ * for each candidate tile length (0: one tile per instance) and returns the
+ * fastest (-1 on error); ms_out[ncand] receives the timings when given. */
+int awamd_batch_tune(void *batch, int precision, const int *cand, int ncand, double *ms_out);
 int awamd_batch_path(void *batch, int i, int *path);
 int awamd_batch_alpha(void *batch, int i, double *alpha);
 int awamd_batch_norms(void *batch, int i, double *norms);
@@ -177,6 +181,24 @@ long long awamd_batch_W(void *batch, int i);
 LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
                            const double *popsizes, double rho, double mu,
                            unsigned long long seed, char *seqs);
+
+/* Continuous-batching server (no counterpart in the reference).  Worker
+ * threads each own a batch with its own HIP streams; a free worker admits
+ * queued requests in arrival order while the estimated device footprint of
+ * the work in flight stays within hbm_budget_gb, so requests arriving while
+ * others run start at once and overlap with them on the GPU.  submit copies
+ * the sequence rows and uniforms and returns an id; wait blocks for it and
+ * fills path[seqlen], the log-likelihood and times3 = {submitted, started,
+ * finished} in seconds on the server clock; it returns 0 on success.  A
+ * request's result does not depend on the batch it ran in. */
+void *awamd_server_new(int workers, double hbm_budget_gb, int precision, int tile_cols,
+                       int max_batch);
+void awamd_server_free(void *server);
+int awamd_server_submit(void *server, LocalTrees *trees, const double *times, int ntimes,
+                        const double *popsizes, double rho, double mu, char **seqs,
+                        int new_chrom, const double *uniforms);
+int awamd_server_wait(void *server, int id, int *path, double *lnl, double *times3);
+double awamd_server_now(void *server);
 
 #ifdef __cplusplus
 }

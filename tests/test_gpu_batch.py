@@ -109,3 +109,18 @@ def test_alone_equals_in_mix(aw, mix, prec, tile):
         assert one.lnl(0) == b.lnl(i), i
         assert np.array_equal(one.path(0), b.path(i)), i
         assert np.array_equal(one.alpha(0), b.alpha(i)), i
+
+
+def test_tile_autotuner(aw, mix):
+    """The tuner times every candidate on the resident tables and returns one
+    of them; a run with it reproduces the untiled thread."""
+    print("tile autotuner", flush=True)
+    b = _batch(aw, mix)
+    b.run(1, 0)
+    ref = [b.path(i) for i in range(len(mix))]
+    best, ms = b.tune(1, (0, 300, 900))
+    assert best in (0, 300, 900)
+    assert all(v > 0 for v in ms.values())
+    b.run(1, best)
+    for i in range(len(mix)):
+        assert np.array_equal(b.path(i), ref[i]), i

@@ -1,0 +1,74 @@
+"""Serving: requests submitted while others run are batched with them and
+overlap on the device; every result equals the same request run alone."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from workloads import instance
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(5, 900, 71), (40, 2500, 72), (12, 3200, 73), (64, 1500, 74), (8, 600, 75),
+         (20, 2000, 76), (30, 1200, 77), (6, 2800, 78), (50, 900, 79), (16, 1800, 80)]
+
+
+def _items(aw):
+    out = []
+    for k, L, seed in CASES:
+        tr, seqs, _, model = instance(aw, k, L, seed)
+        u = np.random.RandomState(seed).uniform(size=L)
+        out.append(dict(k=k, trees=tr, seqs=seqs, model=model, u=u))
+    return out
+
+
+def _alone(aw, it, prec, tile):
+    b = aw.Batch()
+    b.add(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+    b.upload()
+    b.run(prec, tile)
+    return b.path(0), b.lnl(0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_served_equals_alone(aw, prec):
+    print("serve prec=%d" % prec, flush=True)
+    items = _items(aw)
+    tile = 700
+    srv = aw.Server(workers=3, hbm_budget_gb=8.0, precision=prec, tile_cols=tile, max_batch=4)
+    ids = []
+
+    def producer():
+        rng = np.random.RandomState(5)
+        for it in items:
+            ids.append(srv.submit(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"]))
+            time.sleep(rng.exponential(0.003))
+
+    t = threading.Thread(target=producer)
+    t.start()
+    t.join()
+    res = [srv.wait(i) for i in ids]
+    srv.close()
+    for it, r in zip(items, res):
+        path, lnl = _alone(aw, it, prec, tile)
+        assert r["lnl"] == lnl
+        assert np.array_equal(r["path"], path)
+        assert r["t_submit"] <= r["t_start"] <= r["t_done"]
+
+
+def test_budget_admission(aw):
+    """A tiny budget admits one request at a time, still every result."""
+    print("serve tiny budget", flush=True)
+    items = _items(aw)[:4]
+    srv = aw.Server(workers=2, hbm_budget_gb=1e-6, precision=1, tile_cols=0, max_batch=8)
+    ids = [srv.submit(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"]) for it in items]
+    res = [srv.wait(i) for i in ids]
+    srv.close()
+    for it, r in zip(items, res):
+        path, lnl = _alone(aw, it, 1, 0)
+        assert r["lnl"] == lnl and np.array_equal(r["path"], path)
+    # one at a time: no two requests overlap in time
+    spans = sorted((r["t_start"], r["t_done"]) for r in res)
+    for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
+        assert s1 >= e0 - 1e-6
