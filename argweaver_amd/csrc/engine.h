@@ -113,6 +113,7 @@ struct HostInstance {
 struct BatchOptions {
     Precision prec = Precision::FP64;
     int tile_cols = 0;       // traceback tile length (0: one tile per instance)
+    int tile_warm = 256;     // warm-up columns of a speculative tile start
     bool traceback = true;
     bool fixup = true;       // false: speculative traceback tiles only (diagnostics)
 };

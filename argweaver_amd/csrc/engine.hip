@@ -330,7 +330,10 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             cd = tv.D[a];
             cg = tv.E[a] * exp(tv.lnE2[a]);
             ck = tv.E[a] * (low + tv.G2[a] - ct);
-            cj = tv.D[a] * tv.E[a] * (low + tv.G3[a] - ct) + tv.norecombs[a];
+            // the diagonal (stay) entry dominates every state's value: it is
+            // computed as the reference computes tmatrix2, so its rounding
+            // is the reference's
+            cj = tv.get_time(a, a, c, true) - tv.get_time(a, a, 0, false);
             EmitTerms e;
             emit_terms(t, M, sh.mainlen, n, a, e);
             ce = e.inv;
@@ -500,11 +503,20 @@ struct FwLane {
     Real cf[R], cd[R], cg[R], ck[R], cj[R], ce[R];
     int ti[R];
     bool first_in_lane, last_in_lane;
-    Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table
+    Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table (registers)
 };
 
+// The (a,b) table lives in registers (lane b holds column b) for FP32 and
+// in LDS for FP64, where registers are short.
+template <class Real>
+constexpr bool tm_in_lds()
+{
+    return sizeof(Real) == 8;
+}
+
 template <class Real, int R, int NT1>
-__device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Real, R, NT1> &fl)
+__device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Real, R, NT1> &fl,
+                                              Real *TMS)
 {
     constexpr int RW = 64 * R;
     const int lane = threadIdx.x, T1 = d.T - 1;
@@ -532,8 +544,18 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
     // per-time sum and lanes >= T-1 write TF entries nobody reads.
     const AW_GLOBAL double *tmb = gp(d.tm) + (size_t)b * T1 * T1;
     const int lc = min(lane, T1 - 1);
+    if (tm_in_lds<Real>()) {
+        // TMS[a * 32 + b] = P(a -> b); lanes 32..63 duplicate lanes 0..31
+        double t[NT1];
 #pragma unroll
-    for (int a = 0; a < NT1; a++) fl.tmc[a] = (Real)tmb[min(a, T1 - 1) * T1 + lc];
+        for (int a = 0; a < NT1; a++) t[a] = tmb[min(a, T1 - 1) * T1 + lc];
+#pragma unroll
+        for (int a = 0; a < NT1; a++)
+            if (lane < 32) TMS[a * 32 + lane] = (Real)t[a];
+    } else {
+#pragma unroll
+        for (int a = 0; a < NT1; a++) fl.tmc[a] = (Real)tmb[min(a, T1 - 1) * T1 + lc];
+    }
 }
 
 // variant row v, clamped to the last row (an instance without variant
@@ -578,11 +600,12 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     Real *XS = sm + 96;              // RW: previous column in state order (switch)
     Real *NW = XS + RW;              // RW: switch targets
     Real *VS = NW + RW;              // VST rows: staged variant-column emissions
+    Real *TMS = VS + VST * RW;       // 31 x 32: the (a,b) table (FP64)
     AW_GLOBAL Real *alpha = (AW_GLOBAL Real *)d.alpha;
     auto norms = gp(d.norms);
 
     FwLane<Real, R, NT1> fl;
-    fw_load_block<Real, R, NT1>(d, 0, fl);
+    fw_load_block<Real, R, NT1>(d, 0, fl, TMS);
     int b = 0, bend = cp(d.blen)[0];
     if (lane < 32) {
         G2buf[lane] = Real(0);
@@ -678,7 +701,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
                 if (cs != -1) pcs = XS[cs] * sc;
                 b = nb;
                 bend += cp(d.blen)[nb];
-                fw_load_block<Real, R, NT1>(d, b, fl);
+                fw_load_block<Real, R, NT1>(d, b, fl, TMS);
             }
             // emission of the column
             Real e[R];
@@ -750,13 +773,17 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 #pragma unroll
         for (int r = 0; r < R; r++) lds_add(&Gc[fl.ti[r] & INFO_TIME], v[r]);
         lds_order();
-        Real norm = Real(0), t = Real(0);
+        // four interleaved partial sums keep the dependency chains short
+        Real nq[4] = {Real(0), Real(0), Real(0), Real(0)};
+        Real tq[4] = {Real(0), Real(0), Real(0), Real(0)};
 #pragma unroll
         for (int a = 0; a < NT1; a++) {
             const Real g = Gc[a];
-            norm += g;
-            t += g * fl.tmc[a];
+            nq[a & 3] += g;
+            tq[a & 3] += g * (tm_in_lds<Real>() ? TMS[a * 32 + (lane & 31)] : fl.tmc[a]);
         }
+        const Real norm = (nq[0] + nq[1]) + (nq[2] + nq[3]);
+        const Real t = (tq[0] + tq[1]) + (tq[2] + tq[3]);
         // the first column is stored unnormalised (scale 1)
         const Real nsc = (col == 0) ? Real(1) : Real(1) / norm;
         AW_GLOBAL Real *row = alpha + (size_t)col * RW;
@@ -807,6 +834,7 @@ __global__ __launch_bounds__(256) void k_lnl_b(const DevInst *__restrict__ D)
 
 struct Chunk {
     int inst, c0, c1;        // columns [c0, c1) of instance inst
+    int cw;                  // column where the speculative chain starts (>= c1)
     int first, last;         // first / last tile of its instance
 };
 
@@ -938,7 +966,7 @@ __device__ __forceinline__ void row_load(AW_GLOBAL const Real *alpha, int col, R
 // orders loads behind stores, so a per-column prefetch would not hide it).
 template <class Real, int R>
 __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *rows, int hi, int lo,
-                                        int knext, int kinfo, int kblk, int mode)
+                                        int knext, int kinfo, int kblk, int mode, int whi)
 {
     constexpr int RW = 64 * R;
     constexpr int BT = tb_batch<Real, R>();
@@ -996,7 +1024,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *row
             knext = lane_sample<R>(w, tl.ti, S, u[L - 1 - col], kinfo);
             kblk = bc;
             if (mode == 1 && oldp == knext) return col;
-            if (lane == 0) path[col] = knext;
+            if (lane == 0 && col <= whi) path[col] = knext;
             if (col - 1 < bc_start && col - 1 >= lo) {
                 bc--;
                 bc_start = cp(d.bstart)[bc];
@@ -1050,15 +1078,18 @@ __device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mod
         if (threadIdx.x == 0) gp(d.path)[L - 1] = knext;
         hi = L - 2;
     } else if (mode == 0) {
-        knext = tb_column<Real, R>(d, ch.c1, cp(d.u)[L - 1 - ch.c1], kinfo, kblk);
-        hi = ch.c1 - 1;
+        // speculative start: a state sampled from column cw alone, then a
+        // warm-up over [c1, cw) whose samples are not stored; by c1 the
+        // chain has usually coalesced with the true one
+        knext = tb_column<Real, R>(d, ch.cw, cp(d.u)[L - 1 - ch.cw], kinfo, kblk);
+        hi = ch.cw - 1;
     } else {
         knext = snap[blockIdx.x];
         kblk = find_owner(cp(d.bstart), d.B, ch.c1);
         kinfo = gp(d.linfo)[(size_t)kblk * RW + lane_pos(knext, R)];
         hi = ch.c1 - 1;
     }
-    return tb_range<Real, R>(d, tab, rows, hi, ch.c0, knext, kinfo, kblk, mode);
+    return tb_range<Real, R>(d, tab, rows, hi, ch.c0, knext, kinfo, kblk, mode, ch.c1 - 1);
 }
 
 template <class Real, int R>
@@ -1426,7 +1457,8 @@ int Batch::upload()
 // in which a tile is re-sampled from its true boundary state until it
 // coalesces with the stored path; a tile that does not coalesce passes the
 // re-run on to the tile before it.
-static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, bool fixup, RunStats &rs)
+static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
+                           RunStats &rs)
 {
     Streams &c = m.st;
     hipStream_t s = c.main;
@@ -1449,6 +1481,7 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, bool fixup,
             k.c1 = std::min(d.L, (t + 1) * tile);
             k.first = (t == 0);
             k.last = (t == nt - 1);
+            k.cw = k.last ? d.L - 1 : std::min(d.L - 1, k.c1 + std::max(warm, 0));
             chs.push_back(k);
         }
         // (a,b) and (c,a,b) tables, then two row batches
@@ -1521,7 +1554,7 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
     for (int i = 0; i < ncand; i++) {
         RunStats rs;
         (void)hipEventRecord(c.ev[6], c.main);
-        if (traceback_stage(m, fp32, cand[i], true, rs)) return -1;
+        if (traceback_stage(m, fp32, cand[i], BatchOptions().tile_warm, true, rs)) return -1;
         (void)hipEventRecord(c.ev[7], c.main);
         HIP_OK(hipStreamSynchronize(c.main));
         float ms = 0;
@@ -1669,7 +1702,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         t1max[r_index(d.R)] = std::max(t1max[r_index(d.R)], T1);
         size_t &x = shf[r_index(d.R)];
         const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
-        x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R));
+        x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R + 31 * 32));
     }
     if (launch_groups(c, gcnt, goff, [&](int g, int off, int cnt, hipStream_t st) {
             return fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
@@ -1681,7 +1714,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipGetLastError());
 
     RunStats rs;
-    if (opt.traceback && traceback_stage(m, fp32, opt.tile_cols, opt.fixup, rs)) return 1;
+    if (opt.traceback && traceback_stage(m, fp32, opt.tile_cols, opt.tile_warm, opt.fixup, rs))
+        return 1;
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));

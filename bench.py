@@ -100,6 +100,19 @@ def cpu_baseline(A, insts, model, budget_s=15.0):
                    "; oracle forward + traceback"))
 
 
+def read_traffic(path):
+    """HBM bytes per column-state of the FP32 forward from the committed
+    rocprofv3 counter summary (the line 'bytes_per_column_state: X'), or
+    None when it is absent."""
+    try:
+        for line in open(path):
+            if line.strip().startswith("bytes_per_column_state:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def plain_fp64(A, insts, model, budget_s, log):
     """The drop-in path: one instance per launch in FP64 (what one
     arghmm_sample_thread call runs on the GPU), inputs resident, timed
@@ -184,6 +197,10 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=10.0)
     ap.add_argument("--serve-rates", default="50,200", help="Poisson request rates (req/s)")
     ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_lane_batched256_traffic.md"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary giving HBM bytes per column-state")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the threading instances shard across GPUs with no "
@@ -194,6 +211,14 @@ def main():
     import argweaver_amd.argweaverc as A
     if not argweaver_amd.device_available():
         raise SystemExit("no HIP device visible")
+    if args.calib_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read, 1 GiB written")
+        del x, y
     log(A.device_info())
     t0 = time.time()
     insts, model = make_workload(A, args.instances, args.seed)
@@ -256,8 +281,12 @@ def main():
     # step; algorithmic bytes = one FP32 table entry written per column-state
     fwd_ms = mean(st32, "ms_forward")
     achieved = (Wtot * 4) / (fwd_ms * 1e-3)
+    traffic = None
+    bpcs = read_traffic(args.traffic_profile)
+    if bpcs is not None:
+        traffic = bpcs * Wtot   # HBM bytes per launch, scaled from the profiled run
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None,
+                    frac=achieved / HBM_PEAK, traffic=traffic,
                     kernel="k_fw_lane<float>", algorithmic_bytes_per_launch=Wtot * 4,
                     ms_per_launch=fwd_ms)
 
