@@ -813,10 +813,11 @@ __global__ __launch_bounds__(64) void k_fw_lane(const DevInst *__restrict__ D,
 }
 
 // log-likelihood: sum of log column normalisers
-__global__ __launch_bounds__(256) void k_lnl_b(const DevInst *__restrict__ D)
+__global__ __launch_bounds__(256) void k_lnl_b(const DevInst *__restrict__ D,
+                                               const int *__restrict__ list)
 {
     __shared__ double part[256];
-    const DevInst &d = D[blockIdx.x];
+    const DevInst &d = D[list[blockIdx.x]];
     double s = 0.0;
     for (int c = threadIdx.x; c < d.L; c += 256) s += log(gp(d.norms)[c]);
     part[threadIdx.x] = s;
@@ -1156,6 +1157,7 @@ struct Streams {
     hipStream_t main = nullptr;
     hipStream_t side[NRV];
     hipEvent_t fork, join[NRV];
+    hipEvent_t fwend[NRV];        // end of each group's forward (timed)
     hipEvent_t ev[8];
     int create()
     {
@@ -1165,6 +1167,7 @@ struct Streams {
         HIP_OK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         for (auto &e : join) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+        for (auto &e : fwend) HIP_OK(hipEventCreate(&e));
         init = true;
         return 0;
     }
@@ -1174,6 +1177,7 @@ struct Streams {
         (void)hipStreamSynchronize(main);
         for (auto &st : side) (void)hipStreamSynchronize(st);
         for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : fwend) (void)hipEventDestroy(e);
         for (auto &e : join) (void)hipEventDestroy(e);
         (void)hipEventDestroy(fork);
         for (auto &st : side) (void)hipStreamDestroy(st);
@@ -1457,11 +1461,12 @@ int Batch::upload()
 // in which a tile is re-sampled from its true boundary state until it
 // coalesces with the stored path; a tile that does not coalesce passes the
 // re-run on to the tile before it.
-static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
-                           RunStats &rs)
+// Traceback tiles of every instance: tile_cols columns (0: one per
+// instance), grouped by R (cnt/off per group).  Copies the tile list to the
+// device on the main stream; returns the LDS size of the tile kernels.
+static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int *cnt, int *off,
+                      size_t &sht)
 {
-    Streams &c = m.st;
-    hipStream_t s = c.main;
     const int n = (int)m.h.size();
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
     std::vector<Chunk> &chs = m.chunks;
@@ -1469,7 +1474,8 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
     std::vector<int> byr(n);
     std::iota(byr.begin(), byr.end(), 0);
     std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return m.h[a].R < m.h[b].R; });
-    size_t sht = 0;
+    sht = 0;
+    for (int g = 0; g < NRV; g++) cnt[g] = off[g] = 0;
     for (int i : byr) {
         const DevInst &d = m.h[i];
         const int tile = tile_cols > 0 ? tile_cols : d.L;
@@ -1484,11 +1490,13 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
             k.cw = k.last ? d.L - 1 : std::min(d.L - 1, k.c1 + std::max(warm, 0));
             chs.push_back(k);
         }
+        cnt[r_index(d.R)] += nt;
         // (a,b) and (c,a,b) tables, then two row batches
         const int T1 = d.T - 1;
         const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
         sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
+    for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
     const int nch = (int)chs.size();
     for (int pass = 0; pass < 2; pass++) {
         Plan p;
@@ -1498,23 +1506,23 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
         m.d_chunks = p.take<Chunk>(nch);
         if (pass == 0 && m.tiles.ensure(p.used + 4096)) return 1;
     }
-    auto tb = [&](int mode, const std::vector<Chunk> &list, const int *snap, int *flags) -> int {
-        int cnt[NRV] = {0}, off[NRV] = {0};
-        for (const Chunk &k : list) cnt[r_index(m.h[k.inst].R)]++;
-        for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
-        return launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
-            const int *sp = snap ? snap + o : nullptr;
-            int *fp = flags ? flags + o : nullptr;
-            return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st)
-                        : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, mode, sp, fp, sht, st);
-        });
-    };
-    HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
-    if (tb(0, chs, nullptr, nullptr)) return 1;
+    HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice,
+                          m.st.main));
+    return 0;
+}
+
+// Fix-up passes: a tile is re-sampled from its true boundary state until it
+// coalesces with the stored path; a tile that does not coalesce passes the
+// re-run on to the tile before it.
+static int fixup_passes(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
+{
+    Streams &c = m.st;
+    hipStream_t s = c.main;
+    const std::vector<Chunk> &chs = m.chunks;
+    const int nch = (int)chs.size();
     std::vector<int> redo;
     for (int j = 0; j < nch; j++)
         if (!chs[j].last) redo.push_back(j);
-    if (!fixup) redo.clear();
     std::vector<Chunk> sub;
     std::vector<int> flags;
     while (!redo.empty()) {
@@ -1522,9 +1530,18 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
         sub.clear();
         for (int j : redo) sub.push_back(chs[j]);
         const int ns = (int)sub.size();
+        int cnt[NRV] = {0}, off[NRV] = {0};
+        for (const Chunk &k : sub) cnt[r_index(m.h[k.inst].R)]++;
+        for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
         HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
-        if (tb(1, sub, m.d_tsnap, m.d_flags)) return 1;
+        if (launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
+                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, 1, m.d_tsnap + o,
+                                                 m.d_flags + o, sht, st)
+                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, 1, m.d_tsnap + o,
+                                                  m.d_flags + o, sht, st);
+            }))
+            return 1;
         flags.resize(ns);
         HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
@@ -1537,7 +1554,24 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
         }
         redo.swap(next);
     }
+    // the full tile list again on the device (the passes overwrote its front)
+    HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
     return 0;
+}
+
+// The traceback of every instance whose forward table is resident.
+static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
+                           RunStats &rs)
+{
+    int cnt[NRV], off[NRV];
+    size_t sht;
+    if (plan_tiles(m, fp32, tile_cols, warm, cnt, off, sht)) return 1;
+    if (launch_groups(m.st, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
+            return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, 0, nullptr, nullptr, sht, st)
+                        : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, 0, nullptr, nullptr, sht, st);
+        }))
+        return 1;
+    return fixup ? fixup_passes(m, fp32, sht, rs) : 0;
 }
 
 int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out)
@@ -1704,18 +1738,33 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
         x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R + 31 * 32));
     }
+    // Each R group's side stream runs its forward, its log-likelihoods and
+    // its speculative traceback tiles in turn, so groups that finish their
+    // forward early trace back while longer ones are still in the forward.
+    int tcnt[NRV] = {0}, toff[NRV] = {0};
+    size_t sht = 0;
+    if (opt.traceback && plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, tcnt, toff, sht)) return 1;
     if (launch_groups(c, gcnt, goff, [&](int g, int off, int cnt, hipStream_t st) {
-            return fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
-                        : launch_fw_r<double>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g]);
+            if (fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
+                     : launch_fw_r<double>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g]))
+                return 1;
+            HIP_OK(hipEventRecord(c.fwend[g], st));
+            hipLaunchKernelGGL(k_lnl_b, dim3(cnt), dim3(256), 0, st, m.dD, m.d_list + off);
+            HIP_OK(hipGetLastError());
+            if (opt.traceback && tcnt[g]) {
+                const int o = toff[g];
+                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, tcnt[g], 0, nullptr,
+                                                 nullptr, sht, st)
+                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, tcnt[g], 0, nullptr,
+                                                  nullptr, sht, st);
+            }
+            return 0;
         }))
         return 1;
     (void)hipEventRecord(c.ev[4], s);
-    hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD);
-    HIP_OK(hipGetLastError());
 
     RunStats rs;
-    if (opt.traceback && traceback_stage(m, fp32, opt.tile_cols, opt.tile_warm, opt.fixup, rs))
-        return 1;
+    if (opt.traceback && opt.fixup && fixup_passes(m, fp32, sht, rs)) return 1;
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1724,8 +1773,17 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     float ms;
     (void)hipEventElapsedTime(&ms, c.ev[1], c.ev[2]); rs.ms_prep = ms;
     (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]); rs.ms_emit = ms;
-    (void)hipEventElapsedTime(&ms, c.ev[3], c.ev[4]); rs.ms_forward = ms;
-    (void)hipEventElapsedTime(&ms, c.ev[4], c.ev[5]); rs.ms_traceback = ms;
+    // forward: until the last group's forward ends; traceback: from there
+    // to the end of the fix-up passes (the part not hidden behind forwards)
+    float fwd = 0;
+    for (int g = 0; g < NRV; g++) {
+        if (!gcnt[g]) continue;
+        (void)hipEventElapsedTime(&ms, c.ev[3], c.fwend[g]);
+        fwd = std::max(fwd, ms);
+    }
+    rs.ms_forward = fwd;
+    (void)hipEventElapsedTime(&ms, c.ev[3], c.ev[5]);
+    rs.ms_traceback = opt.traceback ? std::max(0.f, ms - fwd) : 0.f;
     (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[5]); rs.ms_total = ms;
     for (int i = 0; i < n; i++) {
         rs.W += m.W[i];
@@ -1823,11 +1881,17 @@ int device_info(char *buf, int n)
     return 0;
 }
 
-// one instance = a batch of one
+// one instance = a batch of one; the batch (its device buffers and streams)
+// is kept per host thread, so repeated calls (every re-threading of
+// resample_arg) reuse the allocations
 int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha, double *lnl,
                  RunStats *stats, Precision prec, const double *prior, int tile_cols)
 {
-    Batch bt;
+    // never destroyed: releasing HIP objects from exit-time destructors can
+    // run after the runtime itself has shut down
+    static thread_local Batch *btp = new Batch();
+    Batch &bt = *btp;
+    bt.clear();
     bt.add(hi, u, prior);
     if (bt.upload()) return 1;
     BatchOptions opt;
