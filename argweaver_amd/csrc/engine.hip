@@ -1897,7 +1897,9 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
     BatchOptions opt;
     opt.prec = prec;
     opt.traceback = (u != nullptr);
-    opt.tile_cols = tile_cols;
+    // long instances trace back in tiles (the thread is the same as with one
+    // tile: fix-up passes re-sample until the chains coalesce)
+    opt.tile_cols = (tile_cols == 0 && hi.ft.seqlen >= 16384) ? 4096 : tile_cols;
     if (bt.run(opt, stats)) return 1;
     if (path && u && bt.copy_path(0, path)) return 1;
     if (alpha && bt.copy_alpha(0, alpha)) return 1;

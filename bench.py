@@ -182,6 +182,36 @@ def serving(A, insts, model, rate, nreq, seed, log, workers=4, budget_gb=96.0, t
     return out
 
 
+def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
+    """Config-5 sample: the MCMC step of resample_arg on a k=100, 5 Mb
+    (500k columns at c=10) ARG under a piecewise-constant N(t): remove a
+    leaf, re-thread it through the drop-in arghmm_sample_thread (forward,
+    traceback, recombinations and ARG surgery).  Times nrethreads steps."""
+    import oracle_bind as O   # time grid helper only
+    times = O.time_points(20, 200e3)
+    pops = np.array([1e4] * 6 + [2e3] * 4 + [1e4] * 10, np.float64)   # a bottleneck
+    rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
+    t0 = time.time()
+    tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed)
+    log("mcmc workload: k=%d L=%d simulated in %.1f s" % (k, L, time.time() - t0))
+    A.set_random_seed(seed)
+    out = []
+    for j in range(nrethreads):
+        # arghmm_sample_thread threads sequence nseqs-1: remove its leaf and
+        # thread it back (one Gibbs step of resample_arg)
+        t0 = time.perf_counter()
+        A.remove_thread(tr, k - 1)
+        W, _ = A.num_column_states(tr, 20)
+        A.sample_thread(tr, seqs, times, pops, rho, mu)
+        dt = time.perf_counter() - t0
+        out.append((dt, W))
+        log("mcmc re-thread %d: W=%.3g, %.2f s" % (j, W, dt))
+    # the first call includes one-time allocations
+    dt, W = out[-1]
+    return dict(k=k, columns=L, popsizes="piecewise 1e4/2e3/1e4", rethread_s=dt,
+                column_states=W, value=W / dt, unit="column-states/s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +227,7 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=10.0)
     ap.add_argument("--serve-rates", default="50,200", help="Poisson request rates (req/s)")
     ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_lane_batched256_traffic.md"),
@@ -276,6 +307,9 @@ def main():
         extra["serving"] = [serving(A, insts, model, float(r), args.serve_requests, args.seed + j,
                                     log, tile=args.tile)
                             for j, r in enumerate(args.serve_rates.split(","))]
+
+    if not args.no_mcmc:
+        extra["mcmc_rethread"] = mcmc_rethread(A, args.seed, log)
 
     # dominant kernel: the FP32 forward (k_fw_lane<float>), one launch per
     # step; algorithmic bytes = one FP32 table entry written per column-state
