@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     for (int p = tid; p < RW; p += NT) {
         const int s = (p & 63) * R + (p >> 6);
         double cf = 0, cd = 0, cg = 0, ck = 0, cj = 0, ce = 0, pr = 0;
-        int inf = INFO_FIRST | INFO_LAST;
+        int inf = INFO_FIRST | INFO_LAST | 31;   // padding: time bin 31, whose TF is 0
         if (s < S) {
             const int n = node_of_state(sh.ni, N, s);
             const int a = sh.ag[n] + s - sh.ni.first[n];
@@ -344,6 +344,15 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             if (b == 0) pr = prior_time(a, sh.L, M);
             inf = a | (a == c ? INFO_FIRST : 0) | (a == h ? INFO_LAST : 0) | INFO_VALID |
                   (c << 11) | (n << 16);
+        }
+        if (d.fp32) {
+            auto c32 = gp(d.lcoef32) + (size_t)b * NPLANE32 * RW;
+            c32[CF * RW + p] = (float)cf;
+            c32[CD * RW + p] = (float)cd;
+            c32[CG * RW + p] = (float)cg;
+            c32[CK * RW + p] = (float)ck;
+            c32[CJ * RW + p] = (float)cj;
+            c32[CE * RW + p] = (float)ce;
         }
         coef[CF * RW + p] = cf;
         coef[CD * RW + p] = cd;
@@ -497,13 +506,21 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
 // arghmm_forward_switch :345-389)
 
 // NT1 is a compile-time bound on the T-1 time bins (19 for the usual 20
-// time points), so the per-time loops are fully unrolled without padding
+// time points), so the per-time loops are fully unrolled without padding.
+//
+// Per block, each lane keeps in registers the factor planes of its R
+// states, two 0/1 factors per state (not the first / not the last state of
+// its branch: they cut the prefix and suffix sums), the LDS byte offset of
+// its time bin (padding states point at bin 31, whose TF stays 0, so they
+// come out 0 with no mask), and whether the lane holds a branch start / end
+// (the carry stops there).
 template <class Real, int R, int NT1>
 struct FwLane {
     Real cf[R], cd[R], cg[R], ck[R], cj[R], ce[R];
-    int ti[R];
-    bool first_in_lane, last_in_lane;
-    Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table (registers)
+    Real nf[R], nl[R];
+    int toff[R];
+    Real m1, m2;
+    Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table (FP32)
 };
 
 // The (a,b) table lives in registers (lane b holds column b) for FP32 and
@@ -514,6 +531,14 @@ constexpr bool tm_in_lds()
     return sizeof(Real) == 8;
 }
 
+template <class Real>
+__device__ __forceinline__ const AW_GLOBAL Real *planes(const DevInst &d, int b, int RW)
+{
+    if (sizeof(Real) == 4)
+        return (const AW_GLOBAL Real *)(gp(d.lcoef32) + (size_t)b * NPLANE32 * RW);
+    return (const AW_GLOBAL Real *)(gp(d.lcoef) + (size_t)b * NCOEF * RW);
+}
+
 template <class Real, int R, int NT1>
 __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Real, R, NT1> &fl,
                                               Real *TMS)
@@ -521,27 +546,35 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
     constexpr int RW = 64 * R;
     const int lane = threadIdx.x, T1 = d.T - 1;
     // uniform bases, per-lane index: loads use the scalar-base address form
-    const AW_GLOBAL double *coef = gp(d.lcoef) + (size_t)b * NCOEF * RW;
+    const AW_GLOBAL Real *coef = planes<Real>(d, b, RW);
     const AW_GLOBAL int *info = gp(d.linfo) + (size_t)b * RW;
-    fl.first_in_lane = false;
-    fl.last_in_lane = false;
+    int ti[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int x = r * 64 + lane;
-        fl.cf[r] = (Real)coef[CF * RW + x];
-        fl.cd[r] = (Real)coef[CD * RW + x];
-        fl.cg[r] = (Real)coef[CG * RW + x];
-        fl.ck[r] = (Real)coef[CK * RW + x];
-        fl.cj[r] = (Real)coef[CJ * RW + x];
-        fl.ce[r] = (Real)coef[CE * RW + x];
-        fl.ti[r] = info[x];
-        fl.first_in_lane |= (fl.ti[r] & INFO_FIRST) != 0;
-        fl.last_in_lane |= (fl.ti[r] & INFO_LAST) != 0;
+        fl.cf[r] = coef[CF * RW + x];
+        fl.cd[r] = coef[CD * RW + x];
+        fl.cg[r] = coef[CG * RW + x];
+        fl.ck[r] = coef[CK * RW + x];
+        fl.cj[r] = coef[CJ * RW + x];
+        fl.ce[r] = coef[CE * RW + x];
+        ti[r] = info[x];
     }
+    bool first = false, last = false;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        fl.nf[r] = (ti[r] & INFO_FIRST) ? Real(0) : Real(1);
+        fl.nl[r] = (ti[r] & INFO_LAST) ? Real(0) : Real(1);
+        fl.toff[r] = (ti[r] & INFO_TIME) * (int)sizeof(Real);
+        first |= (ti[r] & INFO_FIRST) != 0;
+        last |= (ti[r] & INFO_LAST) != 0;
+    }
+    fl.m1 = first ? Real(0) : Real(1);
+    fl.m2 = last ? Real(0) : Real(1);
     // Loaded values go straight into their registers: selecting or copying
     // a value whose load is still in flight makes the compiler wait for it
     // at once.  Indices are clamped instead; entries a >= T-1 meet a zero
-    // per-time sum and lanes >= T-1 write TF entries nobody reads.
+    // per-time sum and lanes >= T-1 compute TF entries nobody stores.
     const AW_GLOBAL double *tmb = gp(d.tm) + (size_t)b * T1 * T1;
     const int lc = min(lane, T1 - 1);
     if (tm_in_lds<Real>()) {
@@ -559,8 +592,7 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
 }
 
 // variant row v, clamped to the last row (an instance without variant
-// columns has one allocated row that is never used), kept as loaded until
-// its column needs it
+// columns has one allocated row that is never used)
 template <int R>
 __device__ __forceinline__ void vrow_load(const DevInst &d, int v, double (&ev)[R])
 {
@@ -576,8 +608,6 @@ __device__ __forceinline__ void lds_add(Real *p, Real v)
     (void)atomicAdd(p, v);
 }
 
-// d is taken by value: its fields stay in scalar registers for the whole
-// loop instead of being re-read from the instance array
 // variant rows staged per LDS refill (at most 16 KB of them)
 template <class Real, int R>
 constexpr int fw_vst()
@@ -590,14 +620,15 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 {
     constexpr int RW = 64 * R;
     constexpr int VST = fw_vst<Real, R>();
+    constexpr int VCH = 2;   // rows per load batch of a refill
     const int lane = threadIdx.x;
     const int T1 = d.T - 1, L = d.L;
     // carry steps: a branch has at most T-1 states, so it reaches back at
     // most ceil((T-2)/R) lanes from the lane holding its last state
     const int K = (T1 - 1 + R - 1) / R;
-    Real *G2buf = sm;                // two 32-entry per-time sum buffers
-    Real *TF = sm + 64;              // 32: other-branch term per target time
-    Real *XS = sm + 96;              // RW: previous column in state order (switch)
+    Real *G = sm;                    // 32: per-time sums of the column being built
+    Real *TF = sm + 32;              // 32: other-branch term per target time (TF[31] = 0)
+    Real *XS = sm + 64;              // RW: previous column in state order (switch)
     Real *NW = XS + RW;              // RW: switch targets
     Real *VS = NW + RW;              // VST rows: staged variant-column emissions
     Real *TMS = VS + VST * RW;       // 31 x 32: the (a,b) table (FP64)
@@ -608,8 +639,8 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     fw_load_block<Real, R, NT1>(d, 0, fl, TMS);
     int b = 0, bend = cp(d.blen)[0];
     if (lane < 32) {
-        G2buf[lane] = Real(0);
-        G2buf[32 + lane] = Real(0);
+        G[lane] = Real(0);
+        TF[lane] = Real(0);
     }
     // Column classes come through scalar loads one column ahead: the LDS
     // waits of a column (lgkmcnt) cover them, while a vector load would
@@ -617,7 +648,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     auto clsp = cp(d.cls);
     int cls_next = clsp[0];
     // Variant rows are consumed in order and staged in LDS, VST at a time:
-    // a refill is one batch of loads and one wait, and the columns read
+    // a refill is a few batches of loads and waits, and the columns read
     // their row from LDS.  (vmcnt counts loads and stores in order, so any
     // per-column HBM load would also wait for the previous column's stores.)
     int vi = 0, vbase = 0, vend = 0;
@@ -625,160 +656,19 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     Real x[R], tfs[R], v[R];
     Real sc = Real(1);   // the previous column is x * sc
 
-    for (int col = 0; col < L; col++) {
-        const int cls = cls_next;
-        cls_next = clsp[min(col + 1, L - 1)];
-        Real *Gc = G2buf + ((col & 1) << 5);
-        Real *Gz = G2buf + (((col + 1) & 1) << 5);
-        // the variant row (column 0 takes no emission but still owns its row)
-        Real ev[R];
-        const bool var = cls >= 0;
-        if (var) {
-            if (vi >= vend) {
-                vbase = vi;
-                vend = min(vi + VST, d.nvar);
-                double buf[VST][R];
+    // per-time sums, normaliser, stored row, other-branch term of the next
+    // column; G is cleared after it is read (LDS operations of one wave run
+    // in order) for the next column's atomics
+    auto finish = [&](int col) {
 #pragma unroll
-                for (int k = 0; k < VST; k++) vrow_load<R>(d, vbase + k, buf[k]);
-#pragma unroll
-                for (int k = 0; k < VST; k++)
-#pragma unroll
-                    for (int r = 0; r < R; r++) VS[k * RW + r * 64 + lane] = (Real)buf[k][r];
-                lds_order();
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) ev[r] = VS[(vi - vbase) * RW + r * 64 + lane];
-            vi++;
-        }
-
-        if (col == 0) {
-            // unnormalised prior, not multiplied by the emission
-            // (calc_state_priors, trans.cpp:785-831)
-            // padding positions of the prior row hold 0
-            const AW_GLOBAL double *pr = gp(d.lprior);
-#pragma unroll
-            for (int r = 0; r < R; r++) v[r] = (Real)pr[r * 64 + lane];
-        } else {
-            const bool sw = (col == bend);
-            Real rr[R], cr[R];
-            Real prs = Real(0), pcs = Real(0);
-            int rs = -1, cs = -1, S2 = 0;
-            if (sw) {
-                // deterministic switch transitions scattered into NW; the
-                // recombination and recoalescence sources are excluded
-                const int nb = b + 1;
-                const int so0 = cp(d.soff)[b], so1 = cp(d.soff)[nb];
-                const int S1 = so1 - so0;
-                S2 = cp(d.soff)[nb + 1] - so1;
-                rs = cp(d.srcs)[2 * nb];
-                cs = cp(d.srcs)[2 * nb + 1];
-                const AW_GLOBAL double *coefn = gp(d.lcoef) + (size_t)nb * NCOEF * RW;
-                const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)nb * RW;
-                int dt[R];
-                Real dpv[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int x = r * 64 + lane;
-                    dt[r] = detn[x];
-                    dpv[r] = (Real)coefn[CDP * RW + x];
-                    rr[r] = (Real)coefn[CRR * RW + x];
-                    cr[r] = (Real)coefn[CCR * RW + x];
-                }
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int s = lane * R + r;
-                    if (s < S1) XS[s] = x[r];
-                }
-                for (int k = lane; k < S2; k += 64) NW[k] = Real(0);
-                lds_order();
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int s = lane * R + r;
-                    if (s < S1 && dt[r] >= 0 && s != rs && s != cs) lds_add(&NW[dt[r]], x[r] * dpv[r]);
-                }
-                lds_order();
-                if (rs != -1) prs = XS[rs] * sc;
-                if (cs != -1) pcs = XS[cs] * sc;
-                b = nb;
-                bend += cp(d.blen)[nb];
-                fw_load_block<Real, R, NT1>(d, b, fl, TMS);
-            }
-            // emission of the column
-            Real e[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) e[r] = var ? ev[r] : (cls == -1) ? fl.ce[r] : Real(1);
-            if (sw) {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int s = lane * R + r;
-                    const Real nw = NW[min(s, RW - 1)];
-                    Real acc = (s < S2) ? nw * sc : Real(0);
-                    if (rs != -1 && rr[r] > Real(0)) acc += prs * rr[r];
-                    if (cs != -1 && cr[r] > Real(0)) acc += pcs * cr[r];
-                    v[r] = (fl.ti[r] & INFO_VALID) ? acc * e[r] : Real(0);
-                }
-            } else {
-                // same-branch prefix sums (younger states) with the carry
-                // from the lanes before, suffix sums (older states) with the
-                // carry from the lanes after.  Branch ends are applied as 0/1
-                // factors converted from the info word: compare-and-select
-                // forms keep one exec mask per slot alive in scalar registers
-                Real P[R], Q[R];
-                Real p = Real(0);
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const Real nf = (Real)(1 - ((fl.ti[r] >> 8) & 1));   // not INFO_FIRST
-                    P[r] = p * nf;
-                    p = P[r] + fl.cf[r] * x[r];
-                }
-                Real q = Real(0);
-#pragma unroll
-                for (int r = R - 1; r >= 0; r--) {
-                    const Real nl = (Real)(1 - ((fl.ti[r] >> 9) & 1));   // not INFO_LAST
-                    Q[r] = q * nl;
-                    q = Q[r] + fl.cd[r] * x[r];
-                }
-                const Real m1 = fl.first_in_lane ? Real(0) : Real(1);
-                const Real m2 = fl.last_in_lane ? Real(0) : Real(1);
-                Real c1 = Real(0), c2 = Real(0);
-                for (int k = 0; k < K; k++) {
-                    c1 = dpp_r<DPP_WAVE_SHR1>(p + m1 * c1);
-                    c2 = dpp_r<DPP_WAVE_SHL1>(q + m2 * c2);
-                }
-                // the carry reaches the slots before the lane's first branch start
-                // (prefix) / after its last branch end (suffix)
-                Real f1 = Real(1);
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    f1 *= (Real)(1 - ((fl.ti[r] >> 8) & 1));
-                    P[r] += c1 * f1;
-                }
-                Real f2 = Real(1);
-#pragma unroll
-                for (int r = R - 1; r >= 0; r--) {
-                    f2 *= (Real)(1 - ((fl.ti[r] >> 9) & 1));
-                    Q[r] += c2 * f2;
-                }
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const Real vm = (Real)((fl.ti[r] >> 10) & 1);   // INFO_VALID
-                    const Real same = (fl.cg[r] * P[r] + fl.ck[r] * Q[r]) + fl.cj[r] * x[r];
-                    v[r] = vm * (e[r] * (tfs[r] + sc * same));
-                }
-            }
-        }
-
-        // per-time sums of the new column, its normaliser, the stored row
-        // and the other-branch term of the next column (padding slots add 0)
-#pragma unroll
-        for (int r = 0; r < R; r++) lds_add(&Gc[fl.ti[r] & INFO_TIME], v[r]);
+        for (int r = 0; r < R; r++) lds_add((Real *)((char *)G + fl.toff[r]), v[r]);
         lds_order();
         // four interleaved partial sums keep the dependency chains short
         Real nq[4] = {Real(0), Real(0), Real(0), Real(0)};
         Real tq[4] = {Real(0), Real(0), Real(0), Real(0)};
 #pragma unroll
         for (int a = 0; a < NT1; a++) {
-            const Real g = Gc[a];
+            const Real g = G[a];
             nq[a & 3] += g;
             tq[a & 3] += g * (tm_in_lds<Real>() ? TMS[a * 32 + (lane & 31)] : fl.tmc[a]);
         }
@@ -790,17 +680,144 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 #pragma unroll
         for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
         if (lane == 0) norms[col] = (col == 0) ? ((L == 1) ? (double)norm : 1.0) : (double)norm;
-        if (lane < 32) {
-            TF[lane] = t * nsc;
-            Gz[lane] = Real(0);
-        }
+        if (lane < T1) TF[lane] = t * nsc;
+        if (lane < 32) G[lane] = Real(0);
         lds_order();
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            tfs[r] = TF[fl.ti[r] & INFO_TIME];
+            tfs[r] = *(const Real *)((const char *)TF + fl.toff[r]);
             x[r] = v[r];
         }
         sc = nsc;
+    };
+
+    // column 0: unnormalised prior, not multiplied by the emission
+    // (calc_state_priors, trans.cpp:785-831); padding positions hold 0
+    {
+        const int cls = cls_next;
+        cls_next = clsp[min(1, L - 1)];
+        if (cls >= 0) vi = 1;   // the column's variant row is never used
+        const AW_GLOBAL double *pr = gp(d.lprior);
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = (Real)pr[r * 64 + lane];
+        finish(0);
+    }
+
+    for (int col = 1; col < L; col++) {
+        const int cls = cls_next;
+        cls_next = clsp[min(col + 1, L - 1)];
+        const bool var = cls >= 0;
+        Real ev[R];
+        if (var) {
+            if (vi >= vend) {
+                vbase = vi;
+                vend = min(vi + VST, d.nvar);
+#pragma unroll
+                for (int k0 = 0; k0 < VST; k0 += VCH) {
+                    // VST need not be a multiple of VCH (FP64, R = 6: 5 rows)
+                    double buf[VCH][R];
+#pragma unroll
+                    for (int k = 0; k < VCH; k++)
+                        if (k0 + k < VST) vrow_load<R>(d, vbase + k0 + k, buf[k]);
+#pragma unroll
+                    for (int k = 0; k < VCH; k++)
+#pragma unroll
+                        for (int r = 0; r < R; r++)
+                            if (k0 + k < VST) VS[(k0 + k) * RW + r * 64 + lane] = (Real)buf[k][r];
+                }
+                lds_order();
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) ev[r] = VS[(vi - vbase) * RW + r * 64 + lane];
+            vi++;
+        }
+
+        if (col == bend) {
+            // switch column: deterministic transitions scattered into NW
+            // (the recombination and recoalescence sources excluded), then
+            // the two dense rows
+            const int nb = b + 1;
+            const int so0 = cp(d.soff)[b], so1 = cp(d.soff)[nb];
+            const int S1 = so1 - so0, S2 = cp(d.soff)[nb + 1] - so1;
+            const int rs = cp(d.srcs)[2 * nb], cs = cp(d.srcs)[2 * nb + 1];
+            const AW_GLOBAL double *coefn = gp(d.lcoef) + (size_t)nb * NCOEF * RW;
+            const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)nb * RW;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int s = lane * R + r;
+                if (s < S1) XS[s] = x[r];
+            }
+            for (int k = lane; k < S2; k += 64) NW[k] = Real(0);
+            lds_order();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int s = lane * R + r;
+                const int dt = detn[r * 64 + lane];
+                const Real dpv = (Real)coefn[CDP * RW + r * 64 + lane];
+                if (s < S1 && dt >= 0 && s != rs && s != cs) lds_add(&NW[dt], x[r] * dpv);
+            }
+            lds_order();
+            const Real prs = (rs != -1) ? XS[rs] * sc : Real(0);
+            const Real pcs = (cs != -1) ? XS[cs] * sc : Real(0);
+            b = nb;
+            bend += cp(d.blen)[nb];
+            fw_load_block<Real, R, NT1>(d, b, fl, TMS);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int s = lane * R + r;
+                const Real rr = (Real)coefn[CRR * RW + r * 64 + lane];
+                const Real cr = (Real)coefn[CCR * RW + r * 64 + lane];
+                const Real e = var ? ev[r] : (cls == -1) ? fl.ce[r] : Real(1);
+                const Real nw = NW[min(s, RW - 1)];
+                Real acc = (s < S2) ? nw * sc : Real(0);
+                if (rs != -1 && rr > Real(0)) acc += prs * rr;
+                if (cs != -1 && cr > Real(0)) acc += pcs * cr;
+                v[r] = (s < S2) ? acc * e : Real(0);
+            }
+        } else {
+            // same-branch prefix sums (younger states) with the carry from
+            // the lanes before, suffix sums (older states) with the carry
+            // from the lanes after
+            Real P[R], Q[R];
+            Real p = Real(0);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                P[r] = p * fl.nf[r];
+                p = fl.cf[r] * x[r] + P[r];
+            }
+            Real q = Real(0);
+#pragma unroll
+            for (int r = R - 1; r >= 0; r--) {
+                Q[r] = q * fl.nl[r];
+                q = fl.cd[r] * x[r] + Q[r];
+            }
+            Real c1 = Real(0), c2 = Real(0);
+            for (int k = 0; k < K; k++) {
+                c1 = dpp_r<DPP_WAVE_SHR1>(fl.m1 * c1 + p);
+                c2 = dpp_r<DPP_WAVE_SHL1>(fl.m2 * c2 + q);
+            }
+            // the carry reaches the slots before the lane's first branch
+            // start (prefix) / after its last branch end (suffix)
+            Real f1 = Real(1);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                f1 *= fl.nf[r];
+                P[r] += c1 * f1;
+            }
+            Real f2 = Real(1);
+#pragma unroll
+            for (int r = R - 1; r >= 0; r--) {
+                f2 *= fl.nl[r];
+                Q[r] += c2 * f2;
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const Real e = var ? ev[r] : (cls == -1) ? fl.ce[r] : Real(1);
+                const Real same = (fl.cg[r] * P[r] + fl.ck[r] * Q[r]) + fl.cj[r] * x[r];
+                v[r] = e * (tfs[r] + sc * same);
+            }
+        }
+        finish(col);
     }
 }
 
@@ -869,7 +886,9 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const int a = tl.ti[r] & INFO_TIME, c = info_age(tl.ti[r]);
+        // padding slots carry time bin 31: clamp so their (zero-weighted)
+        // table reads stay inside the tables
+        const int a = min(tl.ti[r] & INFO_TIME, T1 - 1), c = info_age(tl.ti[r]);
         tl.off1[r] = a * T1;
         tl.off2[r] = TT + tc_base(c, T1) + (a - c) * (T1 - c) - c;
     }
@@ -1641,6 +1660,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             d.srcs = p.take<int>(2 * (size_t)B);
             d.linfo = p.take<int>((size_t)B * RW);
             d.lcoef = p.take<double>((size_t)B * NCOEF * RW);
+            d.fp32 = fp32 ? 1 : 0;
+            d.lcoef32 = fp32 ? p.take<float>((size_t)B * NPLANE32 * RW) : nullptr;
             d.ldet = p.take<int>((size_t)B * RW);
             d.lprior = p.take<double>(RW);
             d.norms = p.take<double>(L);
@@ -1736,7 +1757,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         t1max[r_index(d.R)] = std::max(t1max[r_index(d.R)], T1);
         size_t &x = shf[r_index(d.R)];
         const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
-        x = std::max(x, rsz * (96 + (2 + (size_t)vst) * 64 * (size_t)d.R + 31 * 32));
+        x = std::max(x, rsz * (64 + (2 + (size_t)vst) * 64 * (size_t)d.R + 31 * 32));
     }
     // Each R group's side stream runs its forward, its log-likelihoods and
     // its speculative traceback tiles in turn, so groups that finish their

@@ -36,9 +36,8 @@ def log(*a):
 
 
 def make_workload(A, n, seed, kmin=8, kmax=64, lmin=5000, lmax=200000):
-    import oracle_bind as O   # time grid helper only (no oracle compute)
     rng = np.random.RandomState(seed)
-    times = O.time_points(20, 200e3)
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
     pops = np.full(20, 1e4)
     rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
     insts = []
@@ -187,8 +186,7 @@ def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
     (500k columns at c=10) ARG under a piecewise-constant N(t): remove a
     leaf, re-thread it through the drop-in arghmm_sample_thread (forward,
     traceback, recombinations and ARG surgery).  Times nrethreads steps."""
-    import oracle_bind as O   # time grid helper only
-    times = O.time_points(20, 200e3)
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
     pops = np.array([1e4] * 6 + [2e3] * 4 + [1e4] * 10, np.float64)   # a bottleneck
     rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
     t0 = time.time()
@@ -225,7 +223,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--plain-budget", type=float, default=10.0)
-    ap.add_argument("--serve-rates", default="50,200", help="Poisson request rates (req/s)")
+    ap.add_argument("--serve-rates", default="", help="Poisson request rates (req/s), comma-separated")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
     ap.add_argument("--calib-copy", action="store_true",
