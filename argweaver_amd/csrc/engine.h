@@ -45,7 +45,7 @@ struct DevInst {
     // prep outputs in the lane-run layout, one row of RW per block
     int *linfo;               // [B*RW] time | flags | age(node) << 11 | node << 16
     double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef)
-    float *lcoef32;           // [B*NPLANE32*RW] CF..CE planes in FP32 (FP32 runs)
+    float *lcoef32;           // [B*NCOEF*RW] the planes in FP32 (FP32 runs)
     int fp32;                 // the run is FP32
     int *ldet;                // [B*RW] deterministic switch target of block b-1's states
     double *lprior;           // [RW] first-column prior
@@ -78,8 +78,7 @@ enum : int {
     CE,                       // emission of an invariant column
     CRR, CCR,                 // recombination / recoalescence rows of the switch into b
     CDP,                      // deterministic switch weight of block b-1's states
-    NCOEF,
-    NPLANE32 = CE + 1         // planes kept in FP32 for FP32 runs (CF..CE)
+    NCOEF
 };
 
 // transposed position of state s in a lane-run row

@@ -99,7 +99,8 @@ __device__ inline int find_owner(PT off, int n, X x)
 // shifts) or outside its 16-lane row (row shifts) receives 0.
 
 constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114,
-              DPP_ROW_SHR8 = 0x118, DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+              DPP_ROW_SHR8 = 0x118, DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138,
+              DPP_QUAD_SWAP1 = 0xB1;   // quad_perm [1,0,3,2]: lanes 2k and 2k+1 swap
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int x)
@@ -220,6 +221,17 @@ __device__ inline int node_of_state(const NodeIndex &ni, int n, int k)
         else hi = mid - 1;
     }
     return lo;
+}
+
+// FP32 runs read every coefficient plane of a block in FP32: a copy made
+// once the block's planes are complete
+__device__ __forceinline__ void mirror_planes32(const DevInst &d, int b, int RW, int tid, int NT)
+{
+    if (!d.fp32) return;
+    __syncthreads();
+    const AW_GLOBAL double *src = gp(d.lcoef) + (size_t)b * NCOEF * RW;
+    AW_GLOBAL float *dst = gp(d.lcoef32) + (size_t)b * NCOEF * RW;
+    for (int x = tid; x < NCOEF * RW; x += NT) dst[x] = (float)src[x];
 }
 
 __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
@@ -345,15 +357,6 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             inf = a | (a == c ? INFO_FIRST : 0) | (a == h ? INFO_LAST : 0) | INFO_VALID |
                   (c << 11) | (n << 16);
         }
-        if (d.fp32) {
-            auto c32 = gp(d.lcoef32) + (size_t)b * NPLANE32 * RW;
-            c32[CF * RW + p] = (float)cf;
-            c32[CD * RW + p] = (float)cd;
-            c32[CG * RW + p] = (float)cg;
-            c32[CK * RW + p] = (float)ck;
-            c32[CJ * RW + p] = (float)cj;
-            c32[CE * RW + p] = (float)ce;
-        }
         coef[CF * RW + p] = cf;
         coef[CD * RW + p] = cd;
         coef[CG * RW + p] = cg;
@@ -384,7 +387,10 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         gp(d.tm2s)[(size_t)b * d.TC + x] = tv.get_time(a, bb, c, true);
     }
 
-    if (b == 0) return;
+    if (b == 0) {
+        mirror_planes32(d, b, RW, tid, NT);
+        return;
+    }
     __syncthreads();   // planes cleared before the switch entries are set
 
     // switch matrix from block b-1 (calc_transition_probs_switch), entries
@@ -446,6 +452,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
                                                                spr.ct, sh.cx.rpa, sh.cx.last_len);
         }
     }
+    mirror_planes32(d, b, RW, tid, NT);
 }
 
 //===========================================================================
@@ -510,15 +517,17 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
 //
 // Per block, each lane keeps in registers the factor planes of its R
 // states, two 0/1 factors per state (not the first / not the last state of
-// its branch: they cut the prefix and suffix sums), the LDS byte offset of
-// its time bin (padding states point at bin 31, whose TF stays 0, so they
-// come out 0 with no mask), and whether the lane holds a branch start / end
-// (the carry stops there).
+// its branch: they cut the prefix and suffix sums), and whether the lane
+// holds a branch start / end (the carry stops there).  Per state it also
+// keeps where its value goes in the per-time partial sums and which lane
+// holds the other-branch term of its time; padding states use time bin 31,
+// whose term is 0, so they come out 0 with no mask.
 template <class Real, int R, int NT1>
 struct FwLane {
     Real cf[R], cd[R], cg[R], ck[R], cj[R], ce[R];
     Real nf[R], nl[R];
-    int toff[R];
+    int aoff[R];     // byte offset of the state's slot in the partial sums
+    int boff[R];     // 4 * time: ds_bpermute address of the lane holding TF[time]
     Real m1, m2;
     Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table (FP32)
 };
@@ -531,11 +540,18 @@ constexpr bool tm_in_lds()
     return sizeof(Real) == 8;
 }
 
+// Per-time sums are accumulated in GCOPY copies of the 32 time bins, one
+// copy per group of 64 / GCOPY consecutive lanes, so that the LDS atomics of
+// one instruction seldom meet at one address (colliding atomics are
+// serialised).  Bin a, copy c sits at GP[a * GCOPY + c].  The sums are FP64
+// in both precisions: ds_add_f64 is much faster than ds_add_f32 here.
+constexpr int GCOPY = 16;
+
 template <class Real>
 __device__ __forceinline__ const AW_GLOBAL Real *planes(const DevInst &d, int b, int RW)
 {
     if (sizeof(Real) == 4)
-        return (const AW_GLOBAL Real *)(gp(d.lcoef32) + (size_t)b * NPLANE32 * RW);
+        return (const AW_GLOBAL Real *)(gp(d.lcoef32) + (size_t)b * NCOEF * RW);
     return (const AW_GLOBAL Real *)(gp(d.lcoef) + (size_t)b * NCOEF * RW);
 }
 
@@ -563,9 +579,11 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
     bool first = false, last = false;
 #pragma unroll
     for (int r = 0; r < R; r++) {
+        const int t = ti[r] & INFO_TIME;
         fl.nf[r] = (ti[r] & INFO_FIRST) ? Real(0) : Real(1);
         fl.nl[r] = (ti[r] & INFO_LAST) ? Real(0) : Real(1);
-        fl.toff[r] = (ti[r] & INFO_TIME) * (int)sizeof(Real);
+        fl.aoff[r] = (t * GCOPY + lane / (64 / GCOPY)) * (int)sizeof(double);
+        fl.boff[r] = t * 4;
         first |= (ti[r] & INFO_FIRST) != 0;
         last |= (ti[r] & INFO_LAST) != 0;
     }
@@ -574,7 +592,7 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
     // Loaded values go straight into their registers: selecting or copying
     // a value whose load is still in flight makes the compiler wait for it
     // at once.  Indices are clamped instead; entries a >= T-1 meet a zero
-    // per-time sum and lanes >= T-1 compute TF entries nobody stores.
+    // per-time sum and lanes >= T-1 compute terms nobody uses.
     const AW_GLOBAL double *tmb = gp(d.tm) + (size_t)b * T1 * T1;
     const int lc = min(lane, T1 - 1);
     if (tm_in_lds<Real>()) {
@@ -608,11 +626,49 @@ __device__ __forceinline__ void lds_add(Real *p, Real v)
     (void)atomicAdd(p, v);
 }
 
-// variant rows staged per LDS refill (at most 16 KB of them)
+__device__ __forceinline__ float readlane_r(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double readlane_r(double x, int l)
+{
+    return readlane_d(x, l);
+}
+
+// x of the lane addr / 4 (ds_bpermute)
+__device__ __forceinline__ float bperm_r(float x, int addr)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+__device__ __forceinline__ double bperm_r(double x, int addr)
+{
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// variant rows staged per LDS refill (at most 16 KB of them), and rows per
+// load batch of a refill (bounded by the registers they take)
 template <class Real, int R>
 constexpr int fw_vst()
 {
     return (16384 / (64 * R * (int)sizeof(Real))) < 8 ? (16384 / (64 * R * (int)sizeof(Real))) : 8;
+}
+template <class Real, int R>
+constexpr int fw_vch()
+{
+    return (32 / R) < 2 ? 2 : (32 / R) > fw_vst<Real, R>() ? fw_vst<Real, R>() : (32 / R);
+}
+
+// LDS of the forward kernel, in Reals: partial sums, the previous column in
+// state order, the switch targets (plus one dummy slot per lane), staged
+// variant rows, the (a,b) table
+template <class Real, int R>
+constexpr int fw_lds_reals()
+{
+    return 32 * GCOPY * (int)(sizeof(double) / sizeof(Real)) + 64 * R + (64 * R + 64) +
+           fw_vst<Real, R>() * 64 * R + 31 * 32;
 }
 
 template <class Real, int R, int NT1>
@@ -620,17 +676,16 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 {
     constexpr int RW = 64 * R;
     constexpr int VST = fw_vst<Real, R>();
-    constexpr int VCH = 2;   // rows per load batch of a refill
+    constexpr int VCH = fw_vch<Real, R>();
     const int lane = threadIdx.x;
     const int T1 = d.T - 1, L = d.L;
     // carry steps: a branch has at most T-1 states, so it reaches back at
     // most ceil((T-2)/R) lanes from the lane holding its last state
     const int K = (T1 - 1 + R - 1) / R;
-    Real *G = sm;                    // 32: per-time sums of the column being built
-    Real *TF = sm + 32;              // 32: other-branch term per target time (TF[31] = 0)
-    Real *XS = sm + 64;              // RW: previous column in state order (switch)
-    Real *NW = XS + RW;              // RW: switch targets
-    Real *VS = NW + RW;              // VST rows: staged variant-column emissions
+    double *GP = (double *)sm;       // 32 x GCOPY: per-time partial sums
+    Real *XS = (Real *)(GP + 32 * GCOPY);   // RW: previous column in state order (switch)
+    Real *NW = XS + RW;              // RW + 64: switch targets, then a dummy slot per lane
+    Real *VS = NW + RW + 64;         // VST rows: staged variant-column emissions
     Real *TMS = VS + VST * RW;       // 31 x 32: the (a,b) table (FP64)
     AW_GLOBAL Real *alpha = (AW_GLOBAL Real *)d.alpha;
     auto norms = gp(d.norms);
@@ -638,39 +693,49 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     FwLane<Real, R, NT1> fl;
     fw_load_block<Real, R, NT1>(d, 0, fl, TMS);
     int b = 0, bend = cp(d.blen)[0];
-    if (lane < 32) {
-        G[lane] = Real(0);
-        TF[lane] = Real(0);
-    }
-    // Column classes come through scalar loads one column ahead: the LDS
-    // waits of a column (lgkmcnt) cover them, while a vector load would
-    // order behind the column's HBM stores (vmcnt counts both).
-    auto clsp = cp(d.cls);
-    int cls_next = clsp[0];
-    // Variant rows are consumed in order and staged in LDS, VST at a time:
-    // a refill is a few batches of loads and waits, and the columns read
-    // their row from LDS.  (vmcnt counts loads and stores in order, so any
-    // per-column HBM load would also wait for the previous column's stores.)
+#pragma unroll
+    for (int k = 0; k < 32 * GCOPY / 64; k++) GP[k * 64 + lane] = 0.0;
+    // Column classes of 64 columns at a time, one per lane, read with
+    // readlane; the next 64 are loaded a block ahead, so the one wait per
+    // 64 columns finds them loaded (vmcnt also counts the table stores, so
+    // a per-column load would wait for the previous column's stores).
+    const AW_GLOBAL int *clsg = gp(d.cls);
+    int clsv = clsg[min(lane, L - 1)];
+    int clsn = clsg[min(64 + lane, L - 1)];
+    // Variant rows are consumed in order and staged in LDS, VST at a time,
+    // so that a refill costs a few waits and the columns read their row
+    // from LDS.
     int vi = 0, vbase = 0, vend = 0;
 
     Real x[R], tfs[R], v[R];
     Real sc = Real(1);   // the previous column is x * sc
 
     // per-time sums, normaliser, stored row, other-branch term of the next
-    // column; G is cleared after it is read (LDS operations of one wave run
-    // in order) for the next column's atomics
+    // column
     auto finish = [&](int col) {
 #pragma unroll
-        for (int r = 0; r < R; r++) lds_add((Real *)((char *)G + fl.toff[r]), v[r]);
-        lds_order();
+        for (int r = 0; r < R; r++) lds_add((double *)((char *)GP + fl.aoff[r]), (double)v[r]);
+        // lane l adds the copies of bin l>>1 in its half (l&1) and clears them
+        // (the LDS operations of one wave run in order)
+        double *gq = GP + (lane >> 1) * GCOPY + (lane & 1) * (GCOPY / 2);
+        double q[GCOPY / 2];
+#pragma unroll
+        for (int k = 0; k < GCOPY / 2; k++) q[k] = gq[k];
+#pragma unroll
+        for (int k = 0; k < GCOPY / 2; k++) gq[k] = 0.0;
+#pragma unroll
+        for (int w = 1; w < GCOPY / 2; w *= 2)
+#pragma unroll
+            for (int k = 0; k < GCOPY / 2; k += 2 * w) q[k] += q[k + w];
+        const Real g = (Real)(q[0] + dpp_r<DPP_QUAD_SWAP1>(q[0]));   // lanes 2a, 2a+1: G[a]
         // four interleaved partial sums keep the dependency chains short
         Real nq[4] = {Real(0), Real(0), Real(0), Real(0)};
         Real tq[4] = {Real(0), Real(0), Real(0), Real(0)};
 #pragma unroll
         for (int a = 0; a < NT1; a++) {
-            const Real g = G[a];
-            nq[a & 3] += g;
-            tq[a & 3] += g * (tm_in_lds<Real>() ? TMS[a * 32 + (lane & 31)] : fl.tmc[a]);
+            const Real ga = readlane_r(g, 2 * a);
+            nq[a & 3] += ga;
+            tq[a & 3] += ga * (tm_in_lds<Real>() ? TMS[a * 32 + (lane & 31)] : fl.tmc[a]);
         }
         const Real norm = (nq[0] + nq[1]) + (nq[2] + nq[3]);
         const Real t = (tq[0] + tq[1]) + (tq[2] + tq[3]);
@@ -680,12 +745,11 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 #pragma unroll
         for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
         if (lane == 0) norms[col] = (col == 0) ? ((L == 1) ? (double)norm : 1.0) : (double)norm;
-        if (lane < T1) TF[lane] = t * nsc;
-        if (lane < 32) G[lane] = Real(0);
-        lds_order();
+        // TF[b] lives in lane b; lanes >= T-1 (padding bin 31) hold 0
+        const Real tv = (lane < T1) ? t * nsc : Real(0);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            tfs[r] = *(const Real *)((const char *)TF + fl.toff[r]);
+            tfs[r] = bperm_r(tv, fl.boff[r]);
             x[r] = v[r];
         }
         sc = nsc;
@@ -694,8 +758,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     // column 0: unnormalised prior, not multiplied by the emission
     // (calc_state_priors, trans.cpp:785-831); padding positions hold 0
     {
-        const int cls = cls_next;
-        cls_next = clsp[min(1, L - 1)];
+        const int cls = __builtin_amdgcn_readlane(clsv, 0);
         if (cls >= 0) vi = 1;   // the column's variant row is never used
         const AW_GLOBAL double *pr = gp(d.lprior);
 #pragma unroll
@@ -704,8 +767,11 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     }
 
     for (int col = 1; col < L; col++) {
-        const int cls = cls_next;
-        cls_next = clsp[min(col + 1, L - 1)];
+        if ((col & 63) == 0) {
+            clsv = clsn;
+            clsn = clsg[min(col + 64 + lane, L - 1)];
+        }
+        const int cls = __builtin_amdgcn_readlane(clsv, col & 63);
         const bool var = cls >= 0;
         Real ev[R];
         if (var) {
@@ -714,7 +780,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
                 vend = min(vi + VST, d.nvar);
 #pragma unroll
                 for (int k0 = 0; k0 < VST; k0 += VCH) {
-                    // VST need not be a multiple of VCH (FP64, R = 6: 5 rows)
+                    // VST need not be a multiple of VCH
                     double buf[VCH][R];
 #pragma unroll
                     for (int k = 0; k < VCH; k++)
@@ -725,7 +791,6 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
                         for (int r = 0; r < R; r++)
                             if (k0 + k < VST) VS[(k0 + k) * RW + r * 64 + lane] = (Real)buf[k][r];
                 }
-                lds_order();
             }
 #pragma unroll
             for (int r = 0; r < R; r++) ev[r] = VS[(vi - vbase) * RW + r * 64 + lane];
@@ -735,43 +800,47 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
         if (col == bend) {
             // switch column: deterministic transitions scattered into NW
             // (the recombination and recoalescence sources excluded), then
-            // the two dense rows
+            // the two dense rows.  Every load of the column is issued before
+            // the first use, branch-free, so the column waits once.
             const int nb = b + 1;
             const int so0 = cp(d.soff)[b], so1 = cp(d.soff)[nb];
             const int S1 = so1 - so0, S2 = cp(d.soff)[nb + 1] - so1;
             const int rs = cp(d.srcs)[2 * nb], cs = cp(d.srcs)[2 * nb + 1];
-            const AW_GLOBAL double *coefn = gp(d.lcoef) + (size_t)nb * NCOEF * RW;
+            const AW_GLOBAL Real *swn = planes<Real>(d, nb, RW);
             const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)nb * RW;
+            int dt[R];
+            Real dpv[R], rr[R], cr[R];
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int s = lane * R + r;
-                if (s < S1) XS[s] = x[r];
+                const int x = r * 64 + lane;
+                dt[r] = detn[x];
+                dpv[r] = swn[CDP * RW + x];
+                rr[r] = swn[CRR * RW + x];
+                cr[r] = swn[CCR * RW + x];
             }
-            for (int k = lane; k < S2; k += 64) NW[k] = Real(0);
-            lds_order();
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int s = lane * R + r;
-                const int dt = detn[r * 64 + lane];
-                const Real dpv = (Real)coefn[CDP * RW + r * 64 + lane];
-                if (s < S1 && dt >= 0 && s != rs && s != cs) lds_add(&NW[dt], x[r] * dpv);
+                XS[lane * R + r] = x[r];   // padding slots (s >= S1) are never read
+                NW[r * 64 + lane] = Real(0);
             }
-            lds_order();
-            const Real prs = (rs != -1) ? XS[rs] * sc : Real(0);
-            const Real pcs = (cs != -1) ? XS[cs] * sc : Real(0);
             b = nb;
             bend += cp(d.blen)[nb];
             fw_load_block<Real, R, NT1>(d, b, fl, TMS);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int s = lane * R + r;
-                const Real rr = (Real)coefn[CRR * RW + r * 64 + lane];
-                const Real cr = (Real)coefn[CCR * RW + r * 64 + lane];
+                const bool ok = s < S1 && dt[r] >= 0 && s != rs && s != cs;
+                lds_add(&NW[ok ? dt[r] : RW + lane], ok ? x[r] * dpv[r] : Real(0));
+            }
+            const Real prs = (rs != -1) ? XS[max(rs, 0)] * sc : Real(0);
+            const Real pcs = (cs != -1) ? XS[max(cs, 0)] * sc : Real(0);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int s = lane * R + r;
                 const Real e = var ? ev[r] : (cls == -1) ? fl.ce[r] : Real(1);
-                const Real nw = NW[min(s, RW - 1)];
-                Real acc = (s < S2) ? nw * sc : Real(0);
-                if (rs != -1 && rr > Real(0)) acc += prs * rr;
-                if (cs != -1 && cr > Real(0)) acc += pcs * cr;
+                Real acc = (s < S2) ? NW[s] * sc : Real(0);
+                if (rs != -1 && rr[r] > Real(0)) acc += prs * rr[r];
+                if (cs != -1 && cr[r] > Real(0)) acc += pcs * cr[r];
                 v[r] = (s < S2) ? acc * e : Real(0);
             }
         } else {
@@ -825,7 +894,7 @@ template <class Real, int R, int NT1>
 __global__ __launch_bounds__(64) void k_fw_lane(const DevInst *__restrict__ D,
                                                 const int *__restrict__ list)
 {
-    extern __shared__ double fw_dyn[];
+    extern __shared__ __align__(16) double fw_dyn[];
     fw_instance<Real, R, NT1>(D[list[blockIdx.x]], (Real *)fw_dyn);
 }
 
@@ -1292,6 +1361,20 @@ int launch_tb(const DevInst *dD, const Chunk *chunks, int cnt, int mode, const i
     return 0;
 }
 
+// LDS bytes / size of Real of the forward kernel for R states per lane
+int fw_lds_reals_r(bool fp32, int R)
+{
+    switch (R) {
+    case 1: return fp32 ? fw_lds_reals<float, 1>() : fw_lds_reals<double, 1>();
+    case 2: return fp32 ? fw_lds_reals<float, 2>() : fw_lds_reals<double, 2>();
+    case 4: return fp32 ? fw_lds_reals<float, 4>() : fw_lds_reals<double, 4>();
+    case 6: return fp32 ? fw_lds_reals<float, 6>() : fw_lds_reals<double, 6>();
+    case 8: return fp32 ? fw_lds_reals<float, 8>() : fw_lds_reals<double, 8>();
+    case 12: return fp32 ? fw_lds_reals<float, 12>() : fw_lds_reals<double, 12>();
+    default: return fp32 ? fw_lds_reals<float, 16>() : fw_lds_reals<double, 16>();
+    }
+}
+
 template <class Real>
 int launch_fw_r(int g, const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st,
                 int T1max)
@@ -1661,7 +1744,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             d.linfo = p.take<int>((size_t)B * RW);
             d.lcoef = p.take<double>((size_t)B * NCOEF * RW);
             d.fp32 = fp32 ? 1 : 0;
-            d.lcoef32 = fp32 ? p.take<float>((size_t)B * NPLANE32 * RW) : nullptr;
+            d.lcoef32 = fp32 ? p.take<float>((size_t)B * NCOEF * RW) : nullptr;
             d.ldet = p.take<int>((size_t)B * RW);
             d.lprior = p.take<double>(RW);
             d.norms = p.take<double>(L);
@@ -1756,8 +1839,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         const int T1 = d.T - 1;
         t1max[r_index(d.R)] = std::max(t1max[r_index(d.R)], T1);
         size_t &x = shf[r_index(d.R)];
-        const int vst = std::min(8, 16384 / (64 * d.R * (int)rsz));
-        x = std::max(x, rsz * (64 + (2 + (size_t)vst) * 64 * (size_t)d.R + 31 * 32));
+        x = std::max(x, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
     }
     // Each R group's side stream runs its forward, its log-likelihoods and
     // its speculative traceback tiles in turn, so groups that finish their
