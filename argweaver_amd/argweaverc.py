@@ -131,14 +131,14 @@ awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_doub
 awamd_batch_lnl = _export("awamd_batch_lnl", c_double, [c_void_p, c_int])
 awamd_batch_W = _export("awamd_batch_W", c_longlong, [c_void_p, c_int])
 
-awamd_server_new = _export("awamd_server_new", c_void_p, [c_int, c_double, c_int, c_int, c_int])
+awamd_server_new = _export("awamd_server_new", c_void_p, [c_double, c_int, c_int, c_int])
 awamd_server_free = _export("awamd_server_free", None, [c_void_p])
 awamd_server_submit = _export(
     "awamd_server_submit", c_int,
     [c_void_p, c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p, c_int,
      c_double_p])
 awamd_server_wait = _export("awamd_server_wait", c_int,
-                            [c_void_p, c_int, c_int_p, c_double_p, c_double_p])
+                            [c_void_p, c_int, c_double, c_int_p, c_double_p, c_double_p])
 awamd_server_now = _export("awamd_server_now", c_double, [c_void_p])
 
 _libc = ctypes.CDLL(None)
@@ -449,11 +449,11 @@ class Batch(object):
 
 class Server(object):
     """Continuous-batching server (awamd_server_*): submit() from any thread,
-    wait() for a result; requests are admitted by device-memory budget and
-    overlap on the GPU with the batches already running."""
+    wait() for a result; the requests queued while a batch runs are admitted
+    together, in arrival order, within the device-memory budget."""
 
-    def __init__(self, workers=4, hbm_budget_gb=64.0, precision=1, tile_cols=2000, max_batch=64):
-        self.ptr = awamd_server_new(workers, hbm_budget_gb, precision, tile_cols, max_batch)
+    def __init__(self, hbm_budget_gb=64.0, precision=1, tile_cols=2000, max_batch=64):
+        self.ptr = awamd_server_new(hbm_budget_gb, precision, tile_cols, max_batch)
         if not self.ptr:
             raise RuntimeError("awamd_server_new failed")
         self.L = {}
@@ -468,11 +468,18 @@ class Server(object):
         self.L[i] = cs.seqlen
         return i
 
-    def wait(self, i):
-        path = np.zeros(self.L.pop(i), np.int32)
+    def wait(self, i, timeout=None):
+        """Result of request i; raises TimeoutError after `timeout` seconds
+        (the request stays pending and can be waited for again)."""
+        L = self.L[i]
+        path = np.zeros(L, np.int32)
         lnl = c_double(0)
         t = np.zeros(3, np.float64)
-        rc = awamd_server_wait(self.ptr, i, _iptr(path), ctypes.byref(lnl), _dptr(t))
+        rc = awamd_server_wait(self.ptr, i, float(timeout or 0.0), _iptr(path), ctypes.byref(lnl),
+                               _dptr(t))
+        if rc == -2:
+            raise TimeoutError("request %d not done after %.1f s" % (i, timeout))
+        del self.L[i]
         if rc != 0:
             raise RuntimeError("request %d failed (%d)" % (i, rc))
         return dict(path=path, lnl=lnl.value, t_submit=t[0], t_start=t[1], t_done=t[2])

@@ -699,15 +699,13 @@ extern "C" LocalTrees *awamd_simulate(int k, int seqlen, const double *times, in
 
 extern "C" {
 
-void *awamd_server_new(int workers, double hbm_budget_gb, int precision, int tile_cols,
-                       int max_batch)
+void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch)
 {
     if (!device_available()) {
         fprintf(stderr, "argweaver_amd: awamd_server_new needs a HIP device\n");
         return nullptr;
     }
     ServeOptions o;
-    o.workers = workers;
     o.hbm_budget = (size_t)(hbm_budget_gb * 1073741824.0);
     o.max_batch = max_batch;
     o.run.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
@@ -727,10 +725,13 @@ int awamd_server_submit(void *server, LocalTrees *trees, const double *times, in
     return reinterpret_cast<Server *>(server)->submit(hi, uniforms);
 }
 
-int awamd_server_wait(void *server, int id, int *path, double *lnl, double *times3)
+int awamd_server_wait(void *server, int id, double timeout_s, int *path, double *lnl,
+                      double *times3)
 {
     ServeResult r;
-    if (reinterpret_cast<Server *>(server)->wait(id, &r)) return -1;
+    const int w = reinterpret_cast<Server *>(server)->wait(id, &r, timeout_s);
+    if (w == 2) return -2;
+    if (w) return -1;
     if (path) std::copy(r.path.begin(), r.path.end(), path);
     if (lnl) *lnl = r.lnl;
     if (times3) {

@@ -890,12 +890,25 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     }
 }
 
-template <class Real, int R, int NT1>
-__global__ __launch_bounds__(64) void k_fw_lane(const DevInst *__restrict__ D,
-                                                const int *__restrict__ list)
+// One launch covers every instance of a batch, whatever its states-per-lane
+// count: waves of different R run side by side from the start, with no
+// dependence on how streams map to hardware queues.
+template <class Real, int NT1>
+__global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
+                                               const int *__restrict__ list)
 {
     extern __shared__ __align__(16) double fw_dyn[];
-    fw_instance<Real, R, NT1>(D[list[blockIdx.x]], (Real *)fw_dyn);
+    const DevInst &d = D[list[blockIdx.x]];
+    Real *sm = (Real *)fw_dyn;
+    switch (d.R) {
+    case 1: fw_instance<Real, 1, NT1>(d, sm); break;
+    case 2: fw_instance<Real, 2, NT1>(d, sm); break;
+    case 4: fw_instance<Real, 4, NT1>(d, sm); break;
+    case 6: fw_instance<Real, 6, NT1>(d, sm); break;
+    case 8: fw_instance<Real, 8, NT1>(d, sm); break;
+    case 12: fw_instance<Real, 12, NT1>(d, sm); break;
+    default: fw_instance<Real, 16, NT1>(d, sm); break;
+    }
 }
 
 // log-likelihood: sum of log column normalisers
@@ -1181,15 +1194,26 @@ __device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mod
     return tb_range<Real, R>(d, tab, rows, hi, ch.c0, knext, kinfo, kblk, mode, ch.c1 - 1);
 }
 
-template <class Real, int R>
-__global__ __launch_bounds__(64) void k_tb_lane(const DevInst *__restrict__ D,
-                                                const Chunk *__restrict__ chunks, int mode,
-                                                const int *__restrict__ snap,
-                                                int *__restrict__ flags)
+// every tile of a batch in one launch (see k_fw_all)
+template <class Real>
+__global__ __launch_bounds__(64) void k_tb_all(const DevInst *__restrict__ D,
+                                               const Chunk *__restrict__ chunks, int mode,
+                                               const int *__restrict__ snap,
+                                               int *__restrict__ flags)
 {
     extern __shared__ double tb_dyn[];
     const Chunk ch = chunks[blockIdx.x];
-    const int res = tb_tile<Real, R>(D[ch.inst], ch, mode, snap, tb_dyn);
+    const DevInst &d = D[ch.inst];
+    int res;
+    switch (d.R) {
+    case 1: res = tb_tile<Real, 1>(d, ch, mode, snap, tb_dyn); break;
+    case 2: res = tb_tile<Real, 2>(d, ch, mode, snap, tb_dyn); break;
+    case 4: res = tb_tile<Real, 4>(d, ch, mode, snap, tb_dyn); break;
+    case 6: res = tb_tile<Real, 6>(d, ch, mode, snap, tb_dyn); break;
+    case 8: res = tb_tile<Real, 8>(d, ch, mode, snap, tb_dyn); break;
+    case 12: res = tb_tile<Real, 12>(d, ch, mode, snap, tb_dyn); break;
+    default: res = tb_tile<Real, 16>(d, ch, mode, snap, tb_dyn); break;
+    }
     if (mode == 1 && threadIdx.x == 0) flags[blockIdx.x] = res;
 }
 
@@ -1237,25 +1261,18 @@ int ctx_init()
     return 0;
 }
 
-// Streams and events of one batch: the main stream plus one side stream per
-// R group, all at the default priority.  Each batch owns its own, so batches
-// driven from different host threads overlap on the device.
+// The stream and events of one batch.  Each batch owns its own stream (at
+// the default priority), so batches driven from different host threads
+// overlap on the device.
 struct Streams {
     bool init = false;
     hipStream_t main = nullptr;
-    hipStream_t side[NRV];
-    hipEvent_t fork, join[NRV];
-    hipEvent_t fwend[NRV];        // end of each group's forward (timed)
     hipEvent_t ev[8];
     int create()
     {
         if (init) return 0;
         HIP_OK(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
-        for (auto &st : side) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        for (auto &e : join) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-        for (auto &e : fwend) HIP_OK(hipEventCreate(&e));
         init = true;
         return 0;
     }
@@ -1263,12 +1280,7 @@ struct Streams {
     {
         if (!init) return;
         (void)hipStreamSynchronize(main);
-        for (auto &st : side) (void)hipStreamSynchronize(st);
         for (auto &e : ev) (void)hipEventDestroy(e);
-        for (auto &e : fwend) (void)hipEventDestroy(e);
-        for (auto &e : join) (void)hipEventDestroy(e);
-        (void)hipEventDestroy(fork);
-        for (auto &st : side) (void)hipStreamDestroy(st);
         (void)hipStreamDestroy(main);
         init = false;
     }
@@ -1317,50 +1329,6 @@ int lanes_per_state_r(int Smax)
     return -1;
 }
 
-int r_index(int R)
-{
-    for (int g = 0; g < NRV; g++)
-        if (RVALS[g] == R) return g;
-    return -1;
-}
-
-// Launch per-R work on the side streams, concurrently, joined back into the
-// main stream.  cnt[g] items of group g start at offset off[g].
-template <class F>
-int launch_groups(Streams &c, const int *cnt, const int *off, F fn)
-{
-    HIP_OK(hipEventRecord(c.fork, c.main));
-    for (int g = 0; g < NRV; g++) {
-        if (!cnt[g]) continue;
-        HIP_OK(hipStreamWaitEvent(c.side[g], c.fork, 0));
-        if (fn(g, off[g], cnt[g], c.side[g])) return 1;
-        HIP_OK(hipEventRecord(c.join[g], c.side[g]));
-        HIP_OK(hipStreamWaitEvent(c.main, c.join[g], 0));
-    }
-    return 0;
-}
-
-template <class Real, int R>
-int launch_fw(const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st, int T1max)
-{
-    auto kern = (T1max <= 19) ? k_fw_lane<Real, R, 19> : k_fw_lane<Real, R, MAXT - 1>;
-    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, list);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
-
-template <class Real, int R>
-int launch_tb(const DevInst *dD, const Chunk *chunks, int cnt, int mode, const int *snap,
-              int *flags, size_t sh, hipStream_t st)
-{
-    auto kern = k_tb_lane<Real, R>;
-    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, chunks, mode, snap, flags);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
-
 // LDS bytes / size of Real of the forward kernel for R states per lane
 int fw_lds_reals_r(bool fp32, int R)
 {
@@ -1376,33 +1344,24 @@ int fw_lds_reals_r(bool fp32, int R)
 }
 
 template <class Real>
-int launch_fw_r(int g, const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st,
-                int T1max)
+int launch_fw(const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st, int T1max)
 {
-    switch (RVALS[g]) {
-    case 1: return launch_fw<Real, 1>(dD, list, cnt, sh, st, T1max);
-    case 2: return launch_fw<Real, 2>(dD, list, cnt, sh, st, T1max);
-    case 4: return launch_fw<Real, 4>(dD, list, cnt, sh, st, T1max);
-    case 6: return launch_fw<Real, 6>(dD, list, cnt, sh, st, T1max);
-    case 8: return launch_fw<Real, 8>(dD, list, cnt, sh, st, T1max);
-    case 12: return launch_fw<Real, 12>(dD, list, cnt, sh, st, T1max);
-    default: return launch_fw<Real, 16>(dD, list, cnt, sh, st, T1max);
-    }
+    auto kern = (T1max <= 19) ? k_fw_all<Real, 19> : k_fw_all<Real, MAXT - 1>;
+    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, list);
+    HIP_OK(hipGetLastError());
+    return 0;
 }
 
 template <class Real>
-int launch_tb_r(int g, const DevInst *dD, const Chunk *chunks, int cnt, int mode,
-                const int *snap, int *flags, size_t sh, hipStream_t st)
+int launch_tb(const DevInst *dD, const Chunk *chunks, int cnt, int mode, const int *snap,
+              int *flags, size_t sh, hipStream_t st)
 {
-    switch (RVALS[g]) {
-    case 1: return launch_tb<Real, 1>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    case 2: return launch_tb<Real, 2>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    case 4: return launch_tb<Real, 4>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    case 6: return launch_tb<Real, 6>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    case 8: return launch_tb<Real, 8>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    case 12: return launch_tb<Real, 12>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    default: return launch_tb<Real, 16>(dD, chunks, cnt, mode, snap, flags, sh, st);
-    }
+    auto kern = k_tb_all<Real>;
+    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, chunks, mode, snap, flags);
+    HIP_OK(hipGetLastError());
+    return 0;
 }
 
 }  // namespace
@@ -1566,18 +1525,18 @@ int Batch::upload()
 // Traceback tiles of every instance: tile_cols columns (0: one per
 // instance), grouped by R (cnt/off per group).  Copies the tile list to the
 // device on the main stream; returns the LDS size of the tile kernels.
-static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int *cnt, int *off,
-                      size_t &sht)
+static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t &sht)
 {
     const int n = (int)m.h.size();
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
     std::vector<Chunk> &chs = m.chunks;
     chs.clear();
+    // tiles of the instances with the most states per lane (the slowest
+    // columns) are dispatched first
     std::vector<int> byr(n);
     std::iota(byr.begin(), byr.end(), 0);
-    std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return m.h[a].R < m.h[b].R; });
+    std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return m.h[a].R > m.h[b].R; });
     sht = 0;
-    for (int g = 0; g < NRV; g++) cnt[g] = off[g] = 0;
     for (int i : byr) {
         const DevInst &d = m.h[i];
         const int tile = tile_cols > 0 ? tile_cols : d.L;
@@ -1592,13 +1551,11 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int *c
             k.cw = k.last ? d.L - 1 : std::min(d.L - 1, k.c1 + std::max(warm, 0));
             chs.push_back(k);
         }
-        cnt[r_index(d.R)] += nt;
         // (a,b) and (c,a,b) tables, then two row batches
         const int T1 = d.T - 1;
         const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
         sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
-    for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
     const int nch = (int)chs.size();
     for (int pass = 0; pass < 2; pass++) {
         Plan p;
@@ -1632,17 +1589,10 @@ static int fixup_passes(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
         sub.clear();
         for (int j : redo) sub.push_back(chs[j]);
         const int ns = (int)sub.size();
-        int cnt[NRV] = {0}, off[NRV] = {0};
-        for (const Chunk &k : sub) cnt[r_index(m.h[k.inst].R)]++;
-        for (int g = 1; g < NRV; g++) off[g] = off[g - 1] + cnt[g - 1];
         HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
-        if (launch_groups(c, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
-                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, 1, m.d_tsnap + o,
-                                                 m.d_flags + o, sht, st)
-                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, 1, m.d_tsnap + o,
-                                                  m.d_flags + o, sht, st);
-            }))
+        if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, ns, 1, m.d_tsnap, m.d_flags, sht, s)
+                 : launch_tb<double>(m.dD, m.d_chunks, ns, 1, m.d_tsnap, m.d_flags, sht, s))
             return 1;
         flags.resize(ns);
         HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
@@ -1665,13 +1615,12 @@ static int fixup_passes(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
 static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
                            RunStats &rs)
 {
-    int cnt[NRV], off[NRV];
     size_t sht;
-    if (plan_tiles(m, fp32, tile_cols, warm, cnt, off, sht)) return 1;
-    if (launch_groups(m.st, cnt, off, [&](int g, int o, int cn, hipStream_t st) {
-            return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, cn, 0, nullptr, nullptr, sht, st)
-                        : launch_tb_r<double>(g, m.dD, m.d_chunks + o, cn, 0, nullptr, nullptr, sht, st);
-        }))
+    if (plan_tiles(m, fp32, tile_cols, warm, sht)) return 1;
+    const int nch = (int)m.chunks.size();
+    hipStream_t s = m.st.main;
+    if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s)
+             : launch_tb<double>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s))
         return 1;
     return fixup ? fixup_passes(m, fp32, sht, rs) : 0;
 }
@@ -1820,51 +1769,37 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipGetLastError());
     (void)hipEventRecord(c.ev[3], s);
 
-    // forward: one wave per instance, grouped by R (one kernel per group on
-    // its own stream), longest first inside a group
+    // forward: one wave per instance, all instances in one launch, the
+    // longest first
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (m.h[a].R != m.h[b].R) return m.h[a].R < m.h[b].R;
-        return m.h[a].L > m.h[b].L;
+        const double ca = (double)m.h[a].L * m.h[a].R, cb = (double)m.h[b].L * m.h[b].R;
+        return ca > cb;
     });
-    int gcnt[NRV] = {0}, goff[NRV] = {0};
-    for (int i = 0; i < n; i++) gcnt[r_index(m.h[i].R)]++;
-    for (int g = 1; g < NRV; g++) goff[g] = goff[g - 1] + gcnt[g - 1];
     HIP_OK(hipMemcpyAsync(m.d_list, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    size_t shf[NRV] = {0};
-    int t1max[NRV] = {0};
+    size_t shf = 0;
+    int t1max = 0;
     for (int i = 0; i < n; i++) {
         const DevInst &d = m.h[i];
-        const int T1 = d.T - 1;
-        t1max[r_index(d.R)] = std::max(t1max[r_index(d.R)], T1);
-        size_t &x = shf[r_index(d.R)];
-        x = std::max(x, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
+        t1max = std::max(t1max, d.T - 1);
+        shf = std::max(shf, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
     }
-    // Each R group's side stream runs its forward, its log-likelihoods and
-    // its speculative traceback tiles in turn, so groups that finish their
-    // forward early trace back while longer ones are still in the forward.
-    int tcnt[NRV] = {0}, toff[NRV] = {0};
-    size_t sht = 0;
-    if (opt.traceback && plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, tcnt, toff, sht)) return 1;
-    if (launch_groups(c, gcnt, goff, [&](int g, int off, int cnt, hipStream_t st) {
-            if (fp32 ? launch_fw_r<float>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g])
-                     : launch_fw_r<double>(g, m.dD, m.d_list + off, cnt, shf[g], st, t1max[g]))
-                return 1;
-            HIP_OK(hipEventRecord(c.fwend[g], st));
-            hipLaunchKernelGGL(k_lnl_b, dim3(cnt), dim3(256), 0, st, m.dD, m.d_list + off);
-            HIP_OK(hipGetLastError());
-            if (opt.traceback && tcnt[g]) {
-                const int o = toff[g];
-                return fp32 ? launch_tb_r<float>(g, m.dD, m.d_chunks + o, tcnt[g], 0, nullptr,
-                                                 nullptr, sht, st)
-                            : launch_tb_r<double>(g, m.dD, m.d_chunks + o, tcnt[g], 0, nullptr,
-                                                  nullptr, sht, st);
-            }
-            return 0;
-        }))
+    if (fp32 ? launch_fw<float>(m.dD, m.d_list, n, shf, s, t1max)
+             : launch_fw<double>(m.dD, m.d_list, n, shf, s, t1max))
         return 1;
     (void)hipEventRecord(c.ev[4], s);
+    hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD, m.d_list);
+    HIP_OK(hipGetLastError());
+    // speculative traceback tiles
+    size_t sht = 0;
+    if (opt.traceback) {
+        if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, sht)) return 1;
+        const int nch = (int)m.chunks.size();
+        if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s)
+                 : launch_tb<double>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s))
+            return 1;
+    }
 
     RunStats rs;
     if (opt.traceback && opt.fixup && fixup_passes(m, fp32, sht, rs)) return 1;
@@ -1876,14 +1811,10 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     float ms;
     (void)hipEventElapsedTime(&ms, c.ev[1], c.ev[2]); rs.ms_prep = ms;
     (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]); rs.ms_emit = ms;
-    // forward: until the last group's forward ends; traceback: from there
-    // to the end of the fix-up passes (the part not hidden behind forwards)
+    // forward: until the forward kernel ends; traceback: from there to the
+    // end of the fix-up passes
     float fwd = 0;
-    for (int g = 0; g < NRV; g++) {
-        if (!gcnt[g]) continue;
-        (void)hipEventElapsedTime(&ms, c.ev[3], c.fwend[g]);
-        fwd = std::max(fwd, ms);
-    }
+    (void)hipEventElapsedTime(&fwd, c.ev[3], c.ev[4]);
     rs.ms_forward = fwd;
     (void)hipEventElapsedTime(&ms, c.ev[3], c.ev[5]);
     rs.ms_traceback = opt.traceback ? std::max(0.f, ms - fwd) : 0.f;

@@ -14,9 +14,8 @@ static double wall_seconds()
 
 Server::Server(const ServeOptions &opt) : opt_(opt), t0_(wall_seconds())
 {
-    if (opt_.workers < 1) opt_.workers = 1;
     if (opt_.max_batch < 1) opt_.max_batch = 1;
-    for (int i = 0; i < opt_.workers; i++) threads_.emplace_back(&Server::worker, this);
+    driver_ = std::thread(&Server::worker, this);
 }
 
 Server::~Server()
@@ -26,7 +25,7 @@ Server::~Server()
         stop_ = true;
     }
     cv_work_.notify_all();
-    for (auto &t : threads_) t.join();
+    driver_.join();
     for (Request *r : queue_) delete r;
 }
 
@@ -97,11 +96,16 @@ int Server::submit(const HostInstance &hi, const double *u)
     return r->id;
 }
 
-int Server::wait(int id, ServeResult *out)
+int Server::wait(int id, ServeResult *out, double timeout_s)
 {
     std::unique_lock<std::mutex> lk(mu_);
     if (id < 0 || id >= next_id_) return 1;
-    cv_done_.wait(lk, [&] { return done_.count(id) > 0; });
+    auto ready = [&] { return done_.count(id) > 0; };
+    if (timeout_s > 0) {
+        if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return 2;
+    } else {
+        cv_done_.wait(lk, ready);
+    }
     if (out) *out = std::move(done_[id]);
     done_.erase(id);
     return 0;
@@ -112,23 +116,15 @@ void Server::worker()
     Batch batch;
     std::vector<Request *> mine;
     while (true) {
-        size_t taken = 0;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            // admission in arrival order: a request waits until what is in
-            // flight leaves room for it; one larger than the whole budget runs
-            // when nothing else is in flight
-            cv_work_.wait(lk, [&] {
-                if (stop_ && queue_.empty()) return true;
-                if (queue_.empty()) return false;
-                const size_t b = queue_.front()->bytes;
-                return in_use_ + b <= opt_.hbm_budget || in_use_ == 0;
-            });
-            if (queue_.empty()) return;
+            cv_work_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+            if (queue_.empty()) return;   // stopping, nothing left
+            // admission in arrival order while the batch fits the budget
+            size_t taken = 0;
             while (!queue_.empty() && (int)mine.size() < opt_.max_batch) {
                 const size_t b = queue_.front()->bytes;
-                if (!(in_use_ + b <= opt_.hbm_budget || in_use_ == 0)) break;
-                in_use_ += b;
+                if (!mine.empty() && taken + b > opt_.hbm_budget) break;
                 taken += b;
                 mine.push_back(queue_.front());
                 queue_.pop_front();
@@ -158,7 +154,6 @@ void Server::worker()
         const double t_done = now();
         {
             std::lock_guard<std::mutex> lk(mu_);
-            in_use_ -= taken;
             for (auto &o : res) {
                 o.t_done = t_done;
                 done_[o.id] = std::move(o);
@@ -167,7 +162,6 @@ void Server::worker()
         for (Request *r : mine) delete r;
         mine.clear();
         cv_done_.notify_all();
-        cv_work_.notify_all();
     }
 }
 

@@ -2,13 +2,13 @@
 //
 // Requests (one threading instance each: the ARG with one lineage removed,
 // the sequences, the uniforms of the traceback) are submitted from any host
-// thread.  A pool of workers, each owning a reusable Batch with its own HIP
-// streams, admits queued requests in arrival order as long as the estimated
-// HBM footprint of everything in flight stays within the budget; a worker
-// that becomes free takes every request that fits, so requests arriving
-// while other batches run are admitted at once and their kernels overlap
-// with the running ones on the device.  Each request's result does not
-// depend on the batch it ran in.
+// thread.  One driver thread owns the device work: whenever the GPU is free
+// it takes every queued request, in arrival order, whose estimated HBM
+// footprint still fits the budget (a request larger than the whole budget
+// runs alone) and runs them as one batch, so the requests that arrived while
+// a batch ran share the next launch.  All HIP calls of a server come from
+// its driver thread.  Each request's result does not depend on the batch it
+// ran in.
 #pragma once
 
 #include <condition_variable>
@@ -25,8 +25,7 @@
 namespace aw {
 
 struct ServeOptions {
-    int workers = 4;                       // batches in flight
-    size_t hbm_budget = size_t(64) << 30;  // bytes of device memory for admitted work
+    size_t hbm_budget = size_t(64) << 30;  // bytes of device memory for one batch
     int max_batch = 64;                    // requests per batch
     BatchOptions run;                      // precision and traceback tiles
 };
@@ -50,8 +49,10 @@ public:
 
     // Takes a copy of the sequence rows and the uniforms.  Returns the id.
     int submit(const HostInstance &hi, const double *u);
-    // Blocks until request id is done; the result is removed from the server.
-    int wait(int id, ServeResult *out);
+    // Blocks until request id is done (at most timeout_s seconds when
+    // timeout_s > 0); the result is removed from the server.  Returns 0, 1
+    // for an unknown id, 2 on time-out (the request stays pending).
+    int wait(int id, ServeResult *out, double timeout_s = 0);
     // Seconds since the server started (the clock of the result times).
     double now() const;
     // Device bytes a request needs while it runs (the admission estimate).
@@ -73,10 +74,9 @@ private:
     std::condition_variable cv_work_, cv_done_;
     std::deque<Request *> queue_;
     std::map<int, ServeResult> done_;
-    size_t in_use_ = 0;
     int next_id_ = 0;
     bool stop_ = false;
-    std::vector<std::thread> threads_;
+    std::thread driver_;
     double t0_;
 };
 

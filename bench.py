@@ -142,7 +142,7 @@ def plain_fp64(A, insts, model, budget_s, log):
                 seconds=t_used)
 
 
-def serving(A, insts, model, rate, nreq, seed, log, workers=4, budget_gb=96.0, tile=2000):
+def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, max_batch=64):
     """Poisson arrivals at `rate` requests/s of instances drawn from the
     workload, served with continuous batching (FP32); per-request latency =
     finish - arrival on the server clock."""
@@ -150,8 +150,7 @@ def serving(A, insts, model, rate, nreq, seed, log, workers=4, budget_gb=96.0, t
     rng = np.random.RandomState(seed)
     picks = rng.randint(0, len(insts), size=nreq)
     gaps = rng.exponential(1.0 / rate, size=nreq)
-    srv = A.Server(workers=workers, hbm_budget_gb=budget_gb, precision=1, tile_cols=tile,
-                   max_batch=64)
+    srv = A.Server(hbm_budget_gb=budget_gb, precision=1, tile_cols=tile, max_batch=max_batch)
     ids = [None] * nreq
 
     def producer():
@@ -168,16 +167,22 @@ def serving(A, insts, model, rate, nreq, seed, log, workers=4, budget_gb=96.0, t
     th = threading.Thread(target=producer)
     th.start()
     th.join()
-    res = [srv.wait(i) for i in ids]
+    res = []
+    for j, i in enumerate(ids):
+        res.append(srv.wait(i, timeout=120))
+        if (j + 1) % 64 == 0:
+            log("serving %.0f req/s: %d/%d done at %.1f s" % (rate, j + 1, nreq, srv.now() - t0))
     wall = srv.now() - t0
     srv.close()
     lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
+    nb = len(set(r["t_start"] for r in res))
     W = float(sum(insts[i]["W"] for i in picks))
     out = dict(rate=rate, requests=nreq, p50_ms=float(np.percentile(lat, 50)),
                p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
-               throughput=W / wall, unit="column-states/s", wall_s=wall, workers=workers)
-    log("serving %.0f req/s: p50 %.1f ms, p99 %.1f ms, %.3g column-states/s"
-        % (rate, out["p50_ms"], out["p99_ms"], out["throughput"]))
+               throughput=W / wall, unit="column-states/s", wall_s=wall, batches=nb,
+               max_batch=max_batch)
+    log("serving %.0f req/s: p50 %.1f ms, p99 %.1f ms, %.3g column-states/s, %d batches"
+        % (rate, out["p50_ms"], out["p99_ms"], out["throughput"], nb))
     return out
 
 
@@ -223,7 +228,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--plain-budget", type=float, default=10.0)
-    ap.add_argument("--serve-rates", default="", help="Poisson request rates (req/s), comma-separated")
+    ap.add_argument("--serve-rates", default="50,200,2000",
+                    help="Poisson request rates (req/s), comma-separated")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
     ap.add_argument("--calib-copy", action="store_true",

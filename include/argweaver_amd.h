@@ -182,22 +182,25 @@ LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
                            const double *popsizes, double rho, double mu,
                            unsigned long long seed, char *seqs);
 
-/* Continuous-batching server (no counterpart in the reference).  Worker
- * threads each own a batch with its own HIP streams; a free worker admits
- * queued requests in arrival order while the estimated device footprint of
- * the work in flight stays within hbm_budget_gb, so requests arriving while
- * others run start at once and overlap with them on the GPU.  submit copies
- * the sequence rows and uniforms and returns an id; wait blocks for it and
- * fills path[seqlen], the log-likelihood and times3 = {submitted, started,
- * finished} in seconds on the server clock; it returns 0 on success.  A
+/* Continuous-batching server (no counterpart in the reference).  One
+ * driver thread owns the device work: whenever the GPU is free it takes the
+ * queued requests, in arrival order, while their estimated device footprint
+ * fits hbm_budget_gb (at most max_batch; a request larger than the budget
+ * runs alone) and runs them as one batch, so requests that arrive while a
+ * batch runs share the next launch.  submit copies the sequence rows and
+ * uniforms and returns an id; wait blocks for it (at most timeout_s seconds
+ * when timeout_s > 0) and fills path[seqlen], the log-likelihood and
+ * times3 = {submitted, started, finished} in seconds on the server clock.
+ * It returns 0 on success, -2 on time-out (the request stays pending, wait
+ * again), -1 for an unknown id, else the failed batch's status.  A
  * request's result does not depend on the batch it ran in. */
-void *awamd_server_new(int workers, double hbm_budget_gb, int precision, int tile_cols,
-                       int max_batch);
+void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch);
 void awamd_server_free(void *server);
 int awamd_server_submit(void *server, LocalTrees *trees, const double *times, int ntimes,
                         const double *popsizes, double rho, double mu, char **seqs,
                         int new_chrom, const double *uniforms);
-int awamd_server_wait(void *server, int id, int *path, double *lnl, double *times3);
+int awamd_server_wait(void *server, int id, double timeout_s, int *path, double *lnl,
+                      double *times3);
 double awamd_server_now(void *server);
 
 #ifdef __cplusplus

@@ -1,5 +1,5 @@
-"""Serving: requests submitted while others run are batched with them and
-overlap on the device; every result equals the same request run alone."""
+"""Serving: requests submitted while a batch runs are admitted together into
+the next batch; every result equals the same request run alone."""
 import threading
 import time
 
@@ -36,7 +36,7 @@ def test_served_equals_alone(aw, prec):
     print("serve prec=%d" % prec, flush=True)
     items = _items(aw)
     tile = 700
-    srv = aw.Server(workers=3, hbm_budget_gb=8.0, precision=prec, tile_cols=tile, max_batch=4)
+    srv = aw.Server(hbm_budget_gb=8.0, precision=prec, tile_cols=tile, max_batch=4)
     ids = []
 
     def producer():
@@ -48,7 +48,7 @@ def test_served_equals_alone(aw, prec):
     t = threading.Thread(target=producer)
     t.start()
     t.join()
-    res = [srv.wait(i) for i in ids]
+    res = [srv.wait(i, timeout=60) for i in ids]
     srv.close()
     for it, r in zip(items, res):
         path, lnl = _alone(aw, it, prec, tile)
@@ -61,9 +61,9 @@ def test_budget_admission(aw):
     """A tiny budget admits one request at a time, still every result."""
     print("serve tiny budget", flush=True)
     items = _items(aw)[:4]
-    srv = aw.Server(workers=2, hbm_budget_gb=1e-6, precision=1, tile_cols=0, max_batch=8)
+    srv = aw.Server(hbm_budget_gb=1e-6, precision=1, tile_cols=0, max_batch=8)
     ids = [srv.submit(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"]) for it in items]
-    res = [srv.wait(i) for i in ids]
+    res = [srv.wait(i, timeout=60) for i in ids]
     srv.close()
     for it, r in zip(items, res):
         path, lnl = _alone(aw, it, 1, 0)
