@@ -12,19 +12,23 @@
 //                  lane-run layout (engine.h)
 //   k_var_emit_b   one wave per variant column: Felsenstein partials and the
 //                  emission row over the block's states
-//   k_fw_lane      one wave per instance over all its columns.  Lane l owns
-//                  states l*R .. l*R+R-1 (node-major).  The same-branch part
-//                  of the transition is a per-branch prefix and suffix sum
-//                  (in-lane scan plus a DPP wave-shift carry chain across
-//                  the few lanes a branch spans), the other-branch part goes
-//                  through the per-time sums (LDS float atomics) and the
-//                  (a,b) table, so a column costs O(S + T^2) with two LDS
-//                  round trips; rows are written coalesced.
+//   k_fw_all       one wave per instance over all its columns, every
+//                  instance of a batch in one launch (a switch on R picks
+//                  the instantiation).  Lane l owns states l*R .. l*R+R-1
+//                  (node-major).  The same-branch part of the transition is
+//                  a per-branch prefix and suffix sum (in-lane scan plus a
+//                  DPP wave-shift carry chain across the few lanes a branch
+//                  spans); the other-branch part goes through the per-time
+//                  sums (FP64 LDS atomics into 16 lane-group copies, a
+//                  lane-pair reduction, readlane broadcast) and the (a,b)
+//                  table, the result reaching each state by ds_bpermute; a
+//                  column costs O(S + T^2) and rows are written coalesced.
 //   k_lnl_b        per-instance sum of log normalisers (FP64)
-//   k_tb_lane      one wave per column tile: FP64 weights and CDF (DPP wave
-//                  scan), rows prefetched two columns ahead.  Tiles after
-//                  the last start from a state sampled from their boundary
-//                  column alone; fix-up passes re-run a tile from the true
+//   k_tb_all       one wave per column tile, every tile in one launch: FP64
+//                  weights and CDF (DPP wave scan), rows staged through LDS
+//                  in double-buffered batches.  Tiles before the last start
+//                  from a state sampled from a column past their end and a
+//                  warm-up; fix-up passes re-run a tile from the true
 //                  boundary state until the re-sampled path coalesces with
 //                  the stored one (identical from there on).
 //

@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
-    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_lane_batched256_traffic.md"),
+    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_all_batched256_traffic.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary giving HBM bytes per column-state")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -315,7 +315,7 @@ def main():
     if not args.no_mcmc:
         extra["mcmc_rethread"] = mcmc_rethread(A, args.seed, log)
 
-    # dominant kernel: the FP32 forward (k_fw_lane<float>), one launch per
+    # dominant kernel: the FP32 forward (k_fw_all<float>), one launch per
     # step; algorithmic bytes = one FP32 table entry written per column-state
     fwd_ms = mean(st32, "ms_forward")
     achieved = (Wtot * 4) / (fwd_ms * 1e-3)
@@ -325,7 +325,7 @@ def main():
         traffic = bpcs * Wtot   # HBM bytes per launch, scaled from the profiled run
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK, traffic=traffic,
-                    kernel="k_fw_lane<float>", algorithmic_bytes_per_launch=Wtot * 4,
+                    kernel="k_fw_all<float>", algorithmic_bytes_per_launch=Wtot * 4,
                     ms_per_launch=fwd_ms)
 
     cpu = None
