@@ -124,6 +124,9 @@ awamd_batch_upload = _export("awamd_batch_upload", c_int, [c_void_p])
 awamd_batch_run = _export("awamd_batch_run", c_int, [c_void_p, c_int, c_int, c_int, c_double_p])
 awamd_batch_run_opts = _export(
     "awamd_batch_run_opts", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p])
+awamd_batch_run_ex = _export(
+    "awamd_batch_run_ex", c_int,
+    [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p])
 awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
 awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
 awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
@@ -379,7 +382,7 @@ class Batch(object):
     """Many independent threading instances resident in HBM (awamd_batch_*)."""
 
     STAT_NAMES = ("ms_prep", "ms_emit", "ms_forward", "ms_traceback", "ms_total", "W",
-                  "nvar", "tb_passes", "tb_rerun_cols")
+                  "nvar", "tb_passes", "tb_rerun_cols", "fw_tiles", "fw_reruns")
 
     def __init__(self):
         self.ptr = awamd_batch_new()
@@ -401,12 +404,14 @@ class Batch(object):
             raise RuntimeError("awamd_batch_upload failed")
         self._keep = []
 
-    def run(self, precision=0, tile_cols=0, traceback=True, fixup=True):
+    def run(self, precision=0, tile_cols=0, traceback=True, fixup=True, fw_tile=0, fw_warm=0,
+            fw_tol=0.0):
         """precision 0: FP64 table; 1: FP32 table with FP64 normalisers and
-        traceback CDF.  tile_cols: traceback tile length (0: whole instance)."""
+        traceback CDF.  tile_cols: traceback tile length (0: whole instance).
+        fw_tile / fw_warm / fw_tol: FP32 forward tiles (awamd_batch_run_ex)."""
         st = np.zeros(len(self.STAT_NAMES), np.float64)
-        if awamd_batch_run_opts(self.ptr, precision, tile_cols, int(traceback), int(fixup),
-                                _dptr(st)) != 0:
+        if awamd_batch_run_ex(self.ptr, precision, tile_cols, int(traceback), int(fixup),
+                              int(fw_tile), int(fw_warm), float(fw_tol), _dptr(st)) != 0:
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
 

@@ -4,6 +4,7 @@
 // also does on the host between HMM passes: flattening the ARG, drawing
 // the traceback's uniforms from the process-global glibc rand() stream,
 // sampling recombination points and splicing the thread into the ARG.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -623,6 +624,29 @@ int awamd_batch_run_opts(void *b, int precision, int tile_cols, int traceback, i
 int awamd_batch_run(void *b, int precision, int tile_cols, int traceback, double *stats)
 {
     return awamd_batch_run_opts(b, precision, tile_cols, traceback, 1, stats);
+}
+
+int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int fixup,
+                       int fw_tile, int fw_warm, double fw_tol, double *stats)
+{
+    AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
+    BatchOptions opt;
+    opt.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
+    opt.tile_cols = tile_cols;
+    opt.traceback = traceback != 0;
+    opt.fixup = fixup != 0;
+    opt.fw_tile = fw_tile;
+    if (fw_warm > 0) opt.fw_warm = fw_warm;
+    if (fw_tol > 0) opt.fw_tol = fw_tol;
+    RunStats st;
+    int rc = ab.batch.run(opt, &st);
+    if (stats) {
+        const double v[11] = {st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback, st.ms_total,
+                              (double)st.W, (double)st.nvar, (double)st.tb_passes,
+                              (double)st.tb_rerun_cols, (double)st.fw_tiles, (double)st.fw_reruns};
+        std::copy(v, v + 11, stats);
+    }
+    return rc;
 }
 
 int awamd_batch_tune(void *b, int precision, const int *cand, int ncand, double *ms_out)

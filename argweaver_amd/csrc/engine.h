@@ -99,6 +99,8 @@ struct RunStats {
     int nvar = 0;
     int tb_passes = 0;            // fix-up passes of the tiled traceback
     long long tb_rerun_cols = 0;  // columns re-sampled by those passes
+    int fw_tiles = 0;             // forward tiles launched
+    int fw_reruns = 0;            // forward tiles re-run from their boundary column
 };
 
 // One threading instance described on the host.
@@ -118,6 +120,16 @@ struct BatchOptions {
     int tile_warm = 256;     // warm-up columns of a speculative tile start
     bool traceback = true;
     bool fixup = true;       // false: speculative traceback tiles only (diagnostics)
+    // FP32 forward tiles: instances longer than fw_tile columns are split
+    // into tiles of fw_tile columns (counting the warm-up) that run
+    // concurrently; a tile after the first starts fw_warm columns early
+    // from a uniform vector, and its start is checked against the column
+    // the tile before computes (relative tolerance fw_tol), a tile that
+    // fails being re-run from that column.  0: one tile per instance (FP64
+    // runs always use one).
+    int fw_tile = 0;
+    int fw_warm = 25000;
+    double fw_tol = 1e-5;
 };
 
 // A set of independent threading instances resident in HBM.  upload()

@@ -675,8 +675,35 @@ constexpr int fw_lds_reals()
            fw_vst<Real, R>() * 64 * R + 31 * 32;
 }
 
+// A forward tile: columns [c0, c1) of an instance computed by one wave.
+// mode 0 starts from the prior at column 0.  mode 1 starts at c0 from a
+// uniform vector and stores nothing before cs (a warm-up: the filter
+// forgets its start).  mode 2 starts at c0 from the column held in boundary
+// slot bin (written by the tile before).  A tile with bout >= 0 also
+// computes column c1 into boundary slot bout, for the check of the next
+// tile's start.
+struct FwTile {
+    int inst;
+    int c0, cs, c1;
+    int mode;
+    int bout, bin;
+};
+constexpr int BND_STRIDE = 64 * 16;   // Reals per boundary slot (the largest row)
+
+// wave-wide maximum
+__device__ __forceinline__ double wave_max(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off));
+    return x;
+}
+
+// columns between the checks of a re-run tile against its stored columns
+constexpr int FW_RECHECK = 128;
+
 template <class Real, int R, int NT1>
-__device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
+__device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwTile tile,
+                                            AW_GLOBAL Real *bnd, AW_GLOBAL double *bnorm,
+                                            double tol, AW_GLOBAL int *early)
 {
     constexpr int RW = 64 * R;
     constexpr int VST = fw_vst<Real, R>();
@@ -694,9 +721,13 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     AW_GLOBAL Real *alpha = (AW_GLOBAL Real *)d.alpha;
     auto norms = gp(d.norms);
 
+    const int c0 = tile.c0, cs = tile.cs;
+    // the last column computed: c1 - 1, or c1 into the boundary slot
+    const int clast = tile.bout >= 0 ? tile.c1 : tile.c1 - 1;
     FwLane<Real, R, NT1> fl;
-    fw_load_block<Real, R, NT1>(d, 0, fl, TMS);
-    int b = 0, bend = cp(d.blen)[0];
+    int b = find_owner(cp(d.bstart), d.B, c0);
+    int bend = cp(d.bstart)[b] + cp(d.blen)[b];
+    fw_load_block<Real, R, NT1>(d, b, fl, TMS);
 #pragma unroll
     for (int k = 0; k < 32 * GCOPY / 64; k++) GP[k * 64 + lane] = 0.0;
     // Column classes of 64 columns at a time, one per lane, read with
@@ -704,19 +735,26 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
     // 64 columns finds them loaded (vmcnt also counts the table stores, so
     // a per-column load would wait for the previous column's stores).
     const AW_GLOBAL int *clsg = gp(d.cls);
-    int clsv = clsg[min(lane, L - 1)];
-    int clsn = clsg[min(64 + lane, L - 1)];
-    // Variant rows are consumed in order and staged in LDS, VST at a time,
-    // so that a refill costs a few waits and the columns read their row
-    // from LDS.
-    int vi = 0, vbase = 0, vend = 0;
+    const int cb = c0 & ~63;
+    int clsv = clsg[min(cb + lane, L - 1)];
+    int clsn = clsg[min(cb + 64 + lane, L - 1)];
+    // Variant rows (row index = the column's class) are consumed in order
+    // and staged in LDS, VST at a time, so that a refill costs a few waits
+    // and the columns read their row from LDS.
+    int vbase = 0, vend = 0;
 
     Real x[R], tfs[R], v[R];
     Real sc = Real(1);   // the previous column is x * sc
 
+    // A re-run tile (mode 2) stops once a column it computes agrees with
+    // the column stored by its speculative run (relative tolerance tol):
+    // the filter is contractive, so the stored columns after it, the
+    // normalisers and the boundary column stay within tolerance.
+    bool rejoined = false;
+
     // per-time sums, normaliser, stored row, other-branch term of the next
     // column
-    auto finish = [&](int col) {
+    auto finish = [&](int col, bool first) {
 #pragma unroll
         for (int r = 0; r < R; r++) lds_add((double *)((char *)GP + fl.aoff[r]), (double)v[r]);
         // lane l adds the copies of bin l>>1 in its half (l&1) and clears them
@@ -743,12 +781,36 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
         }
         const Real norm = (nq[0] + nq[1]) + (nq[2] + nq[3]);
         const Real t = (tq[0] + tq[1]) + (tq[2] + tq[3]);
-        // the first column is stored unnormalised (scale 1)
-        const Real nsc = (col == 0) ? Real(1) : Real(1) / norm;
-        AW_GLOBAL Real *row = alpha + (size_t)col * RW;
+        // column 0 is stored unnormalised (scale 1); a restarted tile's first
+        // column is the boundary column as stored by the tile before
+        const Real nsc = (col == 0 || (first && tile.mode == 2)) ? Real(1) : Real(1) / norm;
+        if (col >= cs && col < tile.c1) {
+            AW_GLOBAL Real *row = alpha + (size_t)col * RW;
+            if (tile.mode == 2 && !first && (col - tile.c0) % FW_RECHECK == 0) {
+                double dmax = 0.0, amax = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
-        if (lane == 0) norms[col] = (col == 0) ? ((L == 1) ? (double)norm : 1.0) : (double)norm;
+                for (int r = 0; r < R; r++) {
+                    const double nv = (double)(v[r] * nsc), ov = (double)row[r * 64 + lane];
+                    dmax = fmax(dmax, fabs(nv - ov));
+                    amax = fmax(amax, fabs(nv));
+                }
+                dmax = wave_max(dmax);
+                amax = wave_max(amax);
+                rejoined = dmax <= tol * amax;
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
+            if (lane == 0) {
+                double nv = (col == 0) ? ((L == 1) ? (double)norm : 1.0) : (double)norm;
+                if (first && tile.mode == 2) nv = bnorm[tile.bin];
+                norms[col] = nv;
+            }
+        } else if (col == tile.c1) {
+            AW_GLOBAL Real *row = bnd + (size_t)tile.bout * BND_STRIDE;
+#pragma unroll
+            for (int r = 0; r < R; r++) row[r * 64 + lane] = v[r] * nsc;
+            if (lane == 0) bnorm[tile.bout] = (double)norm;
+        }
         // TF[b] lives in lane b; lanes >= T-1 (padding bin 31) hold 0
         const Real tv = (lane < T1) ? t * nsc : Real(0);
 #pragma unroll
@@ -759,18 +821,25 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
         sc = nsc;
     };
 
-    // column 0: unnormalised prior, not multiplied by the emission
-    // (calc_state_priors, trans.cpp:785-831); padding positions hold 0
-    {
-        const int cls = __builtin_amdgcn_readlane(clsv, 0);
-        if (cls >= 0) vi = 1;   // the column's variant row is never used
+    // first column.  mode 0: the unnormalised prior, not multiplied by the
+    // emission (calc_state_priors, trans.cpp:785-831); padding positions
+    // hold 0.  mode 1: 1 on every state.  mode 2: the boundary column.
+    if (tile.mode == 0) {
         const AW_GLOBAL double *pr = gp(d.lprior);
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = (Real)pr[r * 64 + lane];
-        finish(0);
+    } else if (tile.mode == 1) {
+        const int S = cp(d.soff)[b + 1] - cp(d.soff)[b];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = (lane * R + r < S) ? Real(1) : Real(0);
+    } else {
+        const AW_GLOBAL Real *src = bnd + (size_t)tile.bin * BND_STRIDE;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = src[r * 64 + lane];
     }
+    finish(c0, true);
 
-    for (int col = 1; col < L; col++) {
+    for (int col = c0 + 1; col <= clast; col++) {
         if ((col & 63) == 0) {
             clsv = clsn;
             clsn = clsg[min(col + 64 + lane, L - 1)];
@@ -779,9 +848,9 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
         const bool var = cls >= 0;
         Real ev[R];
         if (var) {
-            if (vi >= vend) {
-                vbase = vi;
-                vend = min(vi + VST, d.nvar);
+            if (cls >= vend) {
+                vbase = cls;
+                vend = min(cls + VST, d.nvar);
 #pragma unroll
                 for (int k0 = 0; k0 < VST; k0 += VCH) {
                     // VST need not be a multiple of VCH
@@ -797,8 +866,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
                 }
             }
 #pragma unroll
-            for (int r = 0; r < R; r++) ev[r] = VS[(vi - vbase) * RW + r * 64 + lane];
-            vi++;
+            for (int r = 0; r < R; r++) ev[r] = VS[(cls - vbase) * RW + r * 64 + lane];
         }
 
         if (col == bend) {
@@ -890,8 +958,10 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
                 v[r] = e * (tfs[r] + sc * same);
             }
         }
-        finish(col);
+        finish(col, false);
+        if (rejoined) break;
     }
+    if (early && lane == 0) *early = rejoined ? 1 : 0;
 }
 
 // One launch covers every instance of a batch, whatever its states-per-lane
@@ -899,19 +969,58 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm)
 // dependence on how streams map to hardware queues.
 template <class Real, int NT1>
 __global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
-                                               const int *__restrict__ list)
+                                               const FwTile *__restrict__ tiles,
+                                               Real *__restrict__ bnd, double *__restrict__ bnorm,
+                                               double tol, int *__restrict__ early)
 {
     extern __shared__ __align__(16) double fw_dyn[];
-    const DevInst &d = D[list[blockIdx.x]];
+    const FwTile t = tiles[blockIdx.x];
+    const DevInst &d = D[t.inst];
     Real *sm = (Real *)fw_dyn;
+    AW_GLOBAL Real *bg = (AW_GLOBAL Real *)bnd;
+    AW_GLOBAL double *ng = (AW_GLOBAL double *)bnorm;
+    AW_GLOBAL int *eg = early ? (AW_GLOBAL int *)(early + blockIdx.x) : nullptr;
     switch (d.R) {
-    case 1: fw_instance<Real, 1, NT1>(d, sm); break;
-    case 2: fw_instance<Real, 2, NT1>(d, sm); break;
-    case 4: fw_instance<Real, 4, NT1>(d, sm); break;
-    case 6: fw_instance<Real, 6, NT1>(d, sm); break;
-    case 8: fw_instance<Real, 8, NT1>(d, sm); break;
-    case 12: fw_instance<Real, 12, NT1>(d, sm); break;
-    default: fw_instance<Real, 16, NT1>(d, sm); break;
+    case 1: fw_instance<Real, 1, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    case 2: fw_instance<Real, 2, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    case 4: fw_instance<Real, 4, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    case 6: fw_instance<Real, 6, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    case 8: fw_instance<Real, 8, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    case 12: fw_instance<Real, 12, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    default: fw_instance<Real, 16, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    }
+}
+
+// Check of a speculative forward tile's start: the boundary column computed
+// by the tile before (slot bin) against the tile's own stored column c0,
+// both normalised.  flag = 1 when they differ by more than tol relative to
+// the column's largest entry, or the normalisers by more than tol.
+template <class Real>
+__global__ __launch_bounds__(64) void k_fw_check(const DevInst *__restrict__ D,
+                                                 const FwTile *__restrict__ tiles,
+                                                 const int *__restrict__ idx,
+                                                 const Real *__restrict__ bnd,
+                                                 const double *__restrict__ bnorm, double tol,
+                                                 int *__restrict__ flags)
+{
+    const FwTile t = tiles[idx[blockIdx.x]];
+    const DevInst &d = D[t.inst];
+    const int RW = 64 * d.R, lane = threadIdx.x;
+    const Real *a = (const Real *)d.alpha + (size_t)t.cs * RW;
+    const Real *q = bnd + (size_t)t.bin * BND_STRIDE;
+    double dmax = 0.0, amax = 0.0;
+    for (int x = lane; x < RW; x += 64) {
+        dmax = fmax(dmax, fabs((double)a[x] - (double)q[x]));
+        amax = fmax(amax, fabs((double)q[x]));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dmax = fmax(dmax, __shfl_xor(dmax, off));
+        amax = fmax(amax, __shfl_xor(amax, off));
+    }
+    if (lane == 0) {
+        const double n0 = gp(d.norms)[t.cs], n1 = bnorm[t.bin];
+        const bool bad = !(dmax <= tol * amax) || !(fabs(n0 - n1) <= tol * fabs(n1));
+        flags[blockIdx.x] = bad ? 1 : 0;
     }
 }
 
@@ -951,10 +1060,11 @@ struct TbLane {
     double dp[R];  // and its weight
 };
 
-// LDS tables of block b: the (a,b) table followed by the (c,a,b) table,
-// copied with 16 loads per lane in flight
-template <int R>
-__device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R> &tl, double *tab)
+// LDS tables of block b: the (a,b) table followed by the (c,a,b) table, in
+// the run's precision, copied with every load of a lane in flight at once
+// (TCP per round)
+template <class Real, int R>
+__device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R> &tl, Real *tab)
 {
     constexpr int RW = 64 * R;
     const int lane = threadIdx.x, T1 = d.T - 1, TT = T1 * T1;
@@ -964,11 +1074,31 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
     const int bn = min(b + 1, d.B - 1);
     const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)bn * RW;
     const AW_GLOBAL double *dpn = gp(d.lcoef) + ((size_t)bn * NCOEF + CDP) * RW;
+    const AW_GLOBAL double *tm = gp(d.tm) + (size_t)b * TT;
+    const AW_GLOBAL double *tm2 = gp(d.tm2s) + (size_t)b * d.TC;
+    const int n = TT + d.TC;
+    constexpr int TCP = sizeof(Real) == 4 ? 48 : 24;
+    for (int base = 0; base < n; base += 64 * TCP) {
+        double t[TCP];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        tl.ti[r] = info[r * 64 + lane];
-        tl.dt[r] = detn[r * 64 + lane];
-        tl.dp[r] = dpn[r * 64 + lane];
+        for (int k = 0; k < TCP; k++) {
+            const int x = min(base + k * 64 + lane, n - 1);
+            const AW_GLOBAL double *src = (x < TT) ? tm + x : tm2 + (x - TT);
+            t[k] = *src;
+        }
+        if (base == 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                tl.ti[r] = info[r * 64 + lane];
+                tl.dt[r] = detn[r * 64 + lane];
+                tl.dp[r] = dpn[r * 64 + lane];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TCP; k++) {
+            const int x = base + k * 64 + lane;
+            if (x < n) tab[x] = (Real)t[k];
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -978,38 +1108,18 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
         tl.off1[r] = a * T1;
         tl.off2[r] = TT + tc_base(c, T1) + (a - c) * (T1 - c) - c;
     }
-    const AW_GLOBAL double *tm = gp(d.tm) + (size_t)b * TT;
-    const AW_GLOBAL double *tm2 = gp(d.tm2s) + (size_t)b * d.TC;
-    const int n = TT + d.TC;
-    constexpr int CP = 16;
-    for (int base = 0; base < n; base += 64 * CP) {
-        double t[CP];
-#pragma unroll
-        for (int k = 0; k < CP; k++) {
-            const int x = min(base + k * 64 + lane, n - 1);
-            const AW_GLOBAL double *src = (x < TT) ? tm + x : tm2 + (x - TT);
-            t[k] = *src;
-        }
-#pragma unroll
-        for (int k = 0; k < CP; k++) {
-            const int x = base + k * 64 + lane;
-            if (x < n) tab[x] = t[k];
-        }
-    }
-    lds_order();
 }
 
 // First index k of the lane-run weights (lane l holds states l*R..l*R+R-1)
 // whose cumulative sum reaches u * total; S-1 if none does.  Returns k and
-// the info word of state k in kinfo.
+// the info word of state k in kinfo.  pre[r] = w[0] + ... + w[r] of the
+// lane.  Branch-free: the owner lane counts the prefixes below the pick.
 template <int R>
-__device__ __forceinline__ int lane_sample(const double (&w)[R], const int (&ti)[R], int S, double u,
-                                  int &kinfo)
+__device__ __forceinline__ int lane_sample(const double (&pre)[R], const int (&ti)[R], int S,
+                                           double u, int &kinfo)
 {
     const int lane = threadIdx.x;
-    double loc = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; r++) loc += w[r];
+    const double loc = pre[R - 1];
     const double inc = wave_incl_scan(loc);
     const double excl = dpp_r<DPP_WAVE_SHR1>(inc);
     const double total = readlane_d(inc, 63);
@@ -1018,24 +1128,12 @@ __device__ __forceinline__ int lane_sample(const double (&w)[R], const int (&ti)
     const unsigned long long m = __ballot(has && excl + loc >= pick);
     const int owner = m ? (int)__ffsll((long long)m) - 1 : (S - 1) / R;
     const int nloc = min(R, S - lane * R);
-    int rsel = nloc - 1;
-    if (m) {
-        double p = excl;
-        bool found = false;
+    int cnt = 0;
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (!found && r < nloc) {
-                p += w[r];
-                if (p >= pick) {
-                    found = true;
-                    rsel = r;
-                }
-            }
-        }
-    }
-    // select ti[rsel] with masks: a compare-and-branch form lets the
-    // optimiser rewrite it as a dynamically indexed load, which puts ti in
-    // scratch memory
+    for (int r = 0; r < R; r++) cnt += (excl + pre[r] < pick) ? 1 : 0;
+    const int rsel = m ? min(cnt, nloc - 1) : nloc - 1;
+    // select ti[rsel] with masks (a compare-and-branch form becomes a
+    // dynamically indexed scratch array)
     int inf = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) inf |= ti[r] & -(int)(r == rsel);
@@ -1070,8 +1168,10 @@ __device__ __forceinline__ void row_load(AW_GLOBAL const Real *alpha, int col, R
 // loads of the next batch are issued when a batch starts and written to
 // LDS when it ends, so the column loop itself never waits on HBM (vmcnt
 // orders loads behind stores, so a per-column prefetch would not hide it).
+// The uniforms and, in mode 1, the stored path come 64 columns at a time,
+// one per lane, read with readlane.
 template <class Real, int R>
-__device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *rows, int hi, int lo,
+__device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows, int hi, int lo,
                                         int knext, int kinfo, int kblk, int mode, int whi)
 {
     constexpr int RW = 64 * R;
@@ -1079,14 +1179,18 @@ __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *row
     const int lane = threadIdx.x, L = d.L;
     AW_GLOBAL const Real *alpha = (AW_GLOBAL const Real *)d.alpha;
     auto path = gp(d.path);
-    auto spath = cp(d.path);   // the stored path of an earlier pass (mode 1)
-    auto u = cp(d.u);
+    const AW_GLOBAL double *ug = gp(d.u);
+    const AW_GLOBAL int *pg = gp(d.path);
     if (hi < lo) return -1;
     int bc = find_owner(cp(d.bstart), d.B, hi);
     int bc_start = cp(d.bstart)[bc];
     int S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
     TbLane<R> tl;
-    tb_load_block<R>(d, bc, tl, tab);
+    tb_load_block<Real, R>(d, bc, tl, tab);
+    // window of 64 columns w0, w0-1, .., w0-63: lane j holds column w0-j
+    int w0 = hi;
+    double uw = ug[min(L - 1 - (w0 - lane), L - 1)];
+    int pw = (mode == 1) ? pg[max(w0 - lane, 0)] : -1;
     Real nxt[BT][R];
 #pragma unroll
     for (int k = 0; k < BT; k++) row_load<Real, R>(alpha, max(hi - k, lo), nxt[k]);
@@ -1101,9 +1205,20 @@ __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *row
         for (int k = 0; k < BT; k++) row_load<Real, R>(alpha, max(top - BT - k, lo), nxt[k]);
         const Real *rb = rows + buf * BT * RW;
         for (int col = top; col >= bot; col--) {
-            const int oldp = (mode == 1) ? spath[col] : -1;
+            if (w0 - col >= 64) {
+                w0 = col;
+                uw = ug[min(L - 1 - (w0 - lane), L - 1)];
+                if (mode == 1) pw = pg[max(w0 - lane, 0)];
+            }
+            const double uval = readlane_d(uw, w0 - col);
+            const int oldp = (mode == 1) ? __builtin_amdgcn_readlane(pw, w0 - col) : -1;
             const Real *row = rb + (top - col) * RW;
-            double w[R];
+            // every load first, then the weights: per-slot conditions around
+            // a load would give each load its own wait
+            Real rv[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) rv[r] = row[r * 64 + lane];
+            double tr[R];
             if (bc != kblk) {
                 // col is the last column of block bc: switch into kblk = bc+1
                 // (TransMatrixSwitch::get, trans.h:201-209)
@@ -1114,20 +1229,25 @@ __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *row
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int s = lane * R + r;
-                    const double tr = (s == cs) ? crk : (s == rs) ? rrk
-                                    : (tl.dt[r] == knext ? tl.dp[r] : 0.0);
-                    w[r] = (tl.ti[r] & INFO_VALID) ? (double)row[r * 64 + lane] * tr : 0.0;
+                    tr[r] = (s == cs) ? crk : (s == rs) ? rrk : (tl.dt[r] == knext ? tl.dp[r] : 0.0);
                 }
             } else {
                 const int nk = info_node(kinfo), bk = kinfo & INFO_TIME;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const bool same = info_node(tl.ti[r]) == nk;
-                    const double tr = tab[(same ? tl.off2[r] : tl.off1[r]) + bk];
-                    w[r] = (tl.ti[r] & INFO_VALID) ? (double)row[r * 64 + lane] * tr : 0.0;
+                    tr[r] = (double)tab[(same ? tl.off2[r] : tl.off1[r]) + bk];
                 }
             }
-            knext = lane_sample<R>(w, tl.ti, S, u[L - 1 - col], kinfo);
+            double pre[R];
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const double vm = (double)((tl.ti[r] >> 10) & 1);   // INFO_VALID
+                acc += vm * ((double)rv[r] * tr[r]);
+                pre[r] = acc;
+            }
+            knext = lane_sample<R>(pre, tl.ti, S, uval, kinfo);
             kblk = bc;
             if (mode == 1 && oldp == knext) return col;
             if (lane == 0 && col <= whi) path[col] = knext;
@@ -1135,7 +1255,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, double *tab, Real *row
                 bc--;
                 bc_start = cp(d.bstart)[bc];
                 S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
-                tb_load_block<R>(d, bc, tl, tab);
+                tb_load_block<Real, R>(d, bc, tl, tab);
             }
         }
         buf ^= 1;
@@ -1162,20 +1282,24 @@ __device__ __forceinline__ int tb_column(const DevInst &d, int col, double uval,
     Real rv[R];
     row_load<Real, R>(alpha, col, rv);
     int ti[R];
-    double w[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) ti[r] = info[r * 64 + threadIdx.x];
+    double pre[R];
+    double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        ti[r] = info[r * 64 + threadIdx.x];
-        w[r] = (ti[r] & INFO_VALID) ? (double)rv[r] : 0.0;
+        acc += (double)((ti[r] >> 10) & 1) * (double)rv[r];
+        pre[r] = acc;
     }
     blk = b;
-    return lane_sample<R>(w, ti, S, uval, kinfo);
+    return lane_sample<R>(pre, ti, S, uval, kinfo);
 }
 
 template <class Real, int R>
-__device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mode, const int *snap, double *tab)
+__device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mode, const int *snap, void *lds)
 {
-    Real *rows = (Real *)(tab + (d.T - 1) * (d.T - 1) + d.TC);
+    Real *tab = (Real *)lds;
+    Real *rows = tab + (d.T - 1) * (d.T - 1) + d.TC;
     constexpr int RW = 64 * R;
     const int L = d.L;
     int knext, kinfo, kblk, hi;
@@ -1348,11 +1472,12 @@ int fw_lds_reals_r(bool fp32, int R)
 }
 
 template <class Real>
-int launch_fw(const DevInst *dD, const int *list, int cnt, size_t sh, hipStream_t st, int T1max)
+int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double *bnorm,
+              size_t sh, hipStream_t st, int T1max, double tol = 0.0, int *early = nullptr)
 {
     auto kern = (T1max <= 19) ? k_fw_all<Real, 19> : k_fw_all<Real, MAXT - 1>;
     HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, list);
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, tiles, (Real *)bnd, bnorm, tol, early);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -1380,13 +1505,17 @@ struct Batch::Impl {
     std::vector<double> lnl;
     std::vector<int> nvar;
     std::vector<Chunk> chunks;
-    DevBuf in, work, tab, aux, vbuf, tiles;
+    DevBuf in, work, tab, aux, vbuf, tiles, fwb;
     Streams st;
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
     int *d_blkoff = nullptr, *d_voff = nullptr, *d_nvar = nullptr, *d_list = nullptr;
     int *d_flags = nullptr, *d_tsnap = nullptr;
     Chunk *d_chunks = nullptr;
+    FwTile *d_fwt = nullptr, *d_fwre = nullptr;
+    char *d_bnd = nullptr;
+    double *d_bnorm = nullptr;
+    int *d_fidx = nullptr, *d_fflag = nullptr;
     bool uploaded = false;
     Precision prec_done = Precision::FP64;
 
@@ -1399,6 +1528,7 @@ struct Batch::Impl {
         aux.release();
         vbuf.release();
         tiles.release();
+        fwb.release();
     }
 };
 
@@ -1558,7 +1688,7 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
         // (a,b) and (c,a,b) tables, then two row batches
         const int T1 = d.T - 1;
         const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
-        sht = std::max(sht, sizeof(double) * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
+        sht = std::max(sht, rsz * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
     const int nch = (int)chs.size();
     for (int pass = 0; pass < 2; pass++) {
@@ -1655,6 +1785,123 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
         }
     }
     return best;
+}
+
+// The forward of every instance.  FP32 runs may split long instances into
+// speculative tiles (BatchOptions::fw_tile); each tile's start is then
+// checked against the boundary column of the tile before, and tiles that
+// fail are re-run from that column until every check passes (a re-run
+// changes the boundary column of the next tile, which is checked again).
+static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
+                         const std::vector<int> &order, size_t shf, int t1max, RunStats &rs)
+{
+    Streams &c = m.st;
+    hipStream_t s = c.main;
+    const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
+    std::vector<FwTile> tl;
+    std::vector<int> next;   // tile index of the following tile of its instance, -1 for none
+    int nslot = 0;
+    const int C = (fp32 && opt.fw_tile > 0) ? opt.fw_tile : 0;
+    const int W = std::max(opt.fw_warm, 1);
+    for (int i : order) {
+        const int L = m.h[i].L;
+        if (!C || L <= C || C <= W) {
+            tl.push_back(FwTile{i, 0, 0, L, 0, -1, -1});
+            next.push_back(-1);
+            continue;
+        }
+        // tile 0: [0, C); then tiles of C - W stored columns after a warm-up
+        // of W columns
+        const int F = C - W;
+        int s0 = 0, e0 = C, prev = -1;
+        while (true) {
+            const bool last = e0 >= L;
+            const int slot = last ? -1 : nslot++;
+            FwTile t;
+            t.inst = i;
+            t.cs = s0;
+            t.c1 = std::min(e0, L);
+            t.c0 = s0 == 0 ? 0 : s0 - W;
+            t.mode = s0 == 0 ? 0 : 1;
+            t.bin = prev;
+            t.bout = slot;
+            if (!next.empty() && prev >= 0) next.back() = (int)tl.size();
+            tl.push_back(t);
+            next.push_back(-1);
+            if (last) break;
+            prev = slot;
+            s0 = e0;
+            e0 = s0 + F;
+        }
+    }
+    const int nt = (int)tl.size();
+    rs.fw_tiles = nt;
+    for (int pass = 0; pass < 2; pass++) {
+        Plan p;
+        p.base = pass ? m.fwb.p : nullptr;
+        m.d_fwt = p.take<FwTile>(nt);
+        m.d_fwre = p.take<FwTile>(nt);
+        m.d_bnd = p.take<char>((size_t)std::max(nslot, 1) * BND_STRIDE * rsz);
+        m.d_bnorm = p.take<double>(std::max(nslot, 1));
+        m.d_fidx = p.take<int>(nt);
+        m.d_fflag = p.take<int>(nt);
+        if (pass == 0 && m.fwb.ensure(p.used + 4096)) return 1;
+    }
+    HIP_OK(hipMemcpyAsync(m.d_fwt, tl.data(), sizeof(FwTile) * nt, hipMemcpyHostToDevice, s));
+    if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max)
+             : launch_fw<double>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max))
+        return 1;
+    // tiles whose start is to be checked: those with a boundary slot in
+    std::vector<int> chk;
+    for (int k = 0; k < nt; k++)
+        if (tl[k].bin >= 0) chk.push_back(k);
+    std::vector<int> flags;
+    while (!chk.empty()) {
+        const int nc = (int)chk.size();
+        HIP_OK(hipMemcpyAsync(m.d_fidx, chk.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+        if (fp32)
+            hipLaunchKernelGGL(k_fw_check<float>, dim3(nc), dim3(64), 0, s, m.dD, m.d_fwt, m.d_fidx,
+                               (const float *)m.d_bnd, m.d_bnorm, opt.fw_tol, m.d_fflag);
+        else
+            hipLaunchKernelGGL(k_fw_check<double>, dim3(nc), dim3(64), 0, s, m.dD, m.d_fwt, m.d_fidx,
+                               (const double *)m.d_bnd, m.d_bnorm, opt.fw_tol, m.d_fflag);
+        HIP_OK(hipGetLastError());
+        flags.resize(nc);
+        HIP_OK(hipMemcpyAsync(flags.data(), m.d_fflag, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        std::vector<int> redo;
+        for (int x = 0; x < nc; x++)
+            if (flags[x]) redo.push_back(chk[x]);
+        if (redo.empty()) break;
+        // re-run from the boundary column, then check the tiles after them
+        std::vector<FwTile> sub;
+        for (int k : redo) {
+            FwTile &t = tl[k];
+            t.mode = 2;
+            t.c0 = t.cs;
+            sub.push_back(t);
+        }
+        rs.fw_reruns += (int)redo.size();
+        // the tile list keeps every tile's index for the checks; the re-run
+        // tiles are launched from a list of their own
+        const int nr = (int)sub.size();
+        HIP_OK(hipMemcpyAsync(m.d_fwt, tl.data(), sizeof(FwTile) * nt, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(m.d_fwre, sub.data(), sizeof(FwTile) * nr, hipMemcpyHostToDevice, s));
+        if (fp32 ? launch_fw<float>(m.dD, m.d_fwre, nr, m.d_bnd, m.d_bnorm, shf, s, t1max,
+                                    opt.fw_tol, m.d_fflag)
+                 : launch_fw<double>(m.dD, m.d_fwre, nr, m.d_bnd, m.d_bnorm, shf, s, t1max,
+                                     opt.fw_tol, m.d_fflag))
+            return 1;
+        // a tile that rejoined its stored columns kept its boundary column;
+        // the tile after one that ran to its end is checked again
+        std::vector<int> early(nr);
+        HIP_OK(hipMemcpyAsync(early.data(), m.d_fflag, sizeof(int) * nr, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        chk.clear();
+        for (int x = 0; x < nr; x++)
+            if (!early[x] && next[redo[x]] >= 0) chk.push_back(next[redo[x]]);
+    }
+    return 0;
 }
 
 int Batch::run(const BatchOptions &opt, RunStats *st)
@@ -1773,8 +2020,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipGetLastError());
     (void)hipEventRecord(c.ev[3], s);
 
-    // forward: one wave per instance, all instances in one launch, the
-    // longest first
+    // forward: one wave per tile, all tiles in one launch, the longest
+    // instances first
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -1789,9 +2036,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         t1max = std::max(t1max, d.T - 1);
         shf = std::max(shf, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
     }
-    if (fp32 ? launch_fw<float>(m.dD, m.d_list, n, shf, s, t1max)
-             : launch_fw<double>(m.dD, m.d_list, n, shf, s, t1max))
-        return 1;
+    RunStats rs;
+    if (forward_stage(m, fp32, opt, order, shf, t1max, rs)) return 1;
     (void)hipEventRecord(c.ev[4], s);
     hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD, m.d_list);
     HIP_OK(hipGetLastError());
@@ -1805,7 +2051,6 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             return 1;
     }
 
-    RunStats rs;
     if (opt.traceback && opt.fixup && fixup_passes(m, fp32, sht, rs)) return 1;
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)

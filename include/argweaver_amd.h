@@ -165,6 +165,15 @@ int awamd_batch_run(void *batch, int precision, int tile_cols, int traceback, do
  * only, for diagnostics) */
 int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int traceback, int fixup,
                          double *stats);
+/* as awamd_batch_run_opts, with FP32 forward tiles: instances longer than
+ * fw_tile columns (0: off; FP64 runs ignore it) run as concurrent tiles of
+ * fw_tile columns; a tile after the first starts fw_warm columns early
+ * (<= 0: 25000) from a uniform vector, its first stored column is checked
+ * against the column computed by the tile before at relative tolerance
+ * fw_tol (<= 0: 1e-5), and a tile that fails is re-run from that column.
+ * stats[11]: the nine of awamd_batch_run, then forward tiles and re-runs. */
+int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback, int fixup,
+                       int fw_tile, int fw_warm, double fw_tol, double *stats);
 /* Traceback-schedule autotuner: runs the forward once, times the traceback
  * for each candidate tile length (0: one tile per instance) and returns the
  * fastest (-1 on error); ms_out[ncand] receives the timings when given. */

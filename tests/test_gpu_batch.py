@@ -80,6 +80,42 @@ def test_batch_fp32_close_to_fp64(aw, mix, tile):
         assert agree >= 0.98, (i, agree)
 
 
+def _fp32_col_err(b, i, it):
+    """max abs difference between b's FP32 columns and the oracle's columns,
+    both normalised per column"""
+    a32 = b.alpha(i)
+    offs = it["inst"].block_offsets()
+    nst = it["inst"].nstates()
+    fw = it["fw"]
+    err = 0.0
+    for blk in range(len(offs) - 1):
+        o0, o1, S = int(offs[blk]), int(offs[blk + 1]), int(nst[blk])
+        ref = fw[o0:o1].reshape(-1, S)
+        got = a32[o0:o1].reshape(-1, S)
+        err = max(err, np.abs(got - ref / ref.sum(axis=1, keepdims=True)).max())
+    return err
+
+
+@pytest.mark.parametrize("warm", [600, 2])
+def test_fp32_forward_tiles(aw, mix, warm):
+    """FP32 forward tiles: every tile start is checked against the column the
+    tile before computes; with a long warm-up the speculative starts pass,
+    with a 2-column warm-up they fail and are re-run from that column.
+    Either way the table, the log-likelihood and the thread stay within the
+    FP32 bounds of the oracle's FP64 results."""
+    print("fp32 forward tiles warm=%d" % warm, flush=True)
+    b = _batch(aw, mix)
+    st = b.run(1, 300, fw_tile=900, fw_warm=warm, fw_tol=1e-5)
+    ntiles = sum(1 + max(0, -(-(m["L"] - 900) // (900 - warm))) for m in mix)
+    assert st["fw_tiles"] == ntiles
+    if warm == 2:
+        assert st["fw_reruns"] >= 1
+    for i, it in enumerate(mix):
+        assert abs(b.lnl(i) - it["lnl"]) <= 1e-6 * abs(it["lnl"]), (i, b.lnl(i), it["lnl"])
+        assert _fp32_col_err(b, i, it) <= 2e-4, i
+        assert np.mean(b.path(i) == it["path"]) >= 0.98, i
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_batch_tiled_traceback_matches_untiled(aw, mix, prec):
     """Tiles with fix-up passes give the same thread as one tile per instance:
