@@ -382,7 +382,7 @@ class Batch(object):
     """Many independent threading instances resident in HBM (awamd_batch_*)."""
 
     STAT_NAMES = ("ms_prep", "ms_emit", "ms_forward", "ms_traceback", "ms_total", "W",
-                  "nvar", "tb_passes", "tb_rerun_cols", "fw_tiles", "fw_reruns")
+                  "nvar", "tb_passes", "tb_rerun_cols", "fw_tiles", "fw_reruns", "ms_fw_main")
 
     def __init__(self):
         self.ptr = awamd_batch_new()

@@ -641,10 +641,11 @@ int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int
     RunStats st;
     int rc = ab.batch.run(opt, &st);
     if (stats) {
-        const double v[11] = {st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback, st.ms_total,
+        const double v[12] = {st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback, st.ms_total,
                               (double)st.W, (double)st.nvar, (double)st.tb_passes,
-                              (double)st.tb_rerun_cols, (double)st.fw_tiles, (double)st.fw_reruns};
-        std::copy(v, v + 11, stats);
+                              (double)st.tb_rerun_cols, (double)st.fw_tiles, (double)st.fw_reruns,
+                              st.ms_fw_main};
+        std::copy(v, v + 12, stats);
     }
     return rc;
 }

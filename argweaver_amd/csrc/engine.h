@@ -99,6 +99,7 @@ struct RunStats {
     int nvar = 0;
     int tb_passes = 0;            // fix-up passes of the tiled traceback
     long long tb_rerun_cols = 0;  // columns re-sampled by those passes
+    double ms_fw_main = 0;        // the main forward launch (without re-runs and checks)
     int fw_tiles = 0;             // forward tiles launched
     int fw_reruns = 0;            // forward tiles re-run from their boundary column
 };

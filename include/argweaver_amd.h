@@ -171,7 +171,8 @@ int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int tracebac
  * (<= 0: 25000) from a uniform vector, its first stored column is checked
  * against the column computed by the tile before at relative tolerance
  * fw_tol (<= 0: 1e-5), and a tile that fails is re-run from that column.
- * stats[11]: the nine of awamd_batch_run, then forward tiles and re-runs. */
+ * stats[12]: the nine of awamd_batch_run, then forward tiles, re-run tiles
+ * and the ms of the main forward launch. */
 int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback, int fixup,
                        int fw_tile, int fw_warm, double fw_tol, double *stats);
 /* Traceback-schedule autotuner: runs the forward once, times the traceback
