@@ -214,7 +214,47 @@ struct PrepShared {
     double mainlen;
     int recombsrc, recoalsrc;
     int node3, rparent;
+    int scan[MAXN];
 };
+
+// Lineage counts (count_lineages) with one thread per node and integer LDS
+// atomics: the single-thread loop is a chain of dependent LDS updates.
+// The counts must be zero on entry; the caller synchronises after.
+__device__ inline void lineages_par(const TreeView &t, int T, Lineages &L, int tid, int NT)
+{
+    for (int i = tid; i < t.n; i += NT) {
+        const int p = t.parent[i];
+        const int page = (p == -1) ? T - 2 : t.age[p];
+        for (int j = t.age[i]; j < page; j++) {
+            atomicAdd(&L.nb[j], 1);
+            atomicAdd(&L.nr[j], 1);
+            atomicAdd(&L.nc[j], 1);
+        }
+        atomicAdd(&L.nr[page], 1);
+        atomicAdd(&L.nc[page], 1);
+        if (p == -1) atomicAdd(&L.nb[page], 1);
+    }
+}
+
+// build_node_index with a block-wide scan (blockDim.x == MAXN)
+__device__ inline void node_index_par(const TreeView &t, int T, NodeIndex &ni, int *scan, int tid)
+{
+    int len = 0;
+    if (tid < t.n) {
+        ni.hi[tid] = t.top(tid, T);
+        len = ni.hi[tid] - t.age[tid] + 1;
+    }
+    scan[tid] = len;
+    __syncthreads();
+    for (int off = 1; off < MAXN; off <<= 1) {
+        const int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    if (tid < t.n) ni.first[tid] = scan[tid] - len;
+    __syncthreads();
+}
 
 __device__ inline int node_of_state(const NodeIndex &ni, int n, int k)
 {
@@ -270,28 +310,39 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     TreeView pt{N, -1, sh.ppar, sh.pch0, sh.pch1, sh.pag};
     Spr spr{0, 0, 0, 0};
     if (b > 0) spr = Spr{gp(d.spr)[4 * b], gp(d.spr)[4 * b + 1], gp(d.spr)[4 * b + 2], gp(d.spr)[4 * b + 3]};
+    // roots, zeroed lineage counts
+    for (int i = tid; i < N; i += NT) {
+        if (sh.par[i] == -1) sh.root = i;
+        if (b > 0 && sh.ppar[i] == -1) sh.proot = i;
+    }
+    for (int i = tid; i < MAXT; i += NT) {
+        sh.L.nb[i] = sh.L.nr[i] = sh.L.nc[i] = 0;
+        sh.PL.nb[i] = sh.PL.nr[i] = sh.PL.nc[i] = 0;
+    }
+    __syncthreads();
+    t.root = sh.root;
+    pt.root = (b > 0) ? sh.proot : -1;
+    lineages_par(t, T, sh.L, tid, NT);
+    if (b > 0) lineages_par(pt, T, sh.PL, tid, NT);
+    node_index_par(t, T, sh.ni, sh.scan, tid);
+    if (b > 0) node_index_par(pt, T, sh.pni, sh.scan, tid);
     if (tid == 0) {
-        int r = 0;
-        for (int i = 0; i < N; i++) if (sh.par[i] == -1) r = i;
-        sh.root = r;
-        t.root = r;
-        count_lineages(t, T, sh.L);
+        sh.L.nb[T - 1] = 1;
+        sh.PL.nb[T - 1] = 1;
+    }
+    __syncthreads();
+    // the remaining sequential pieces, one wave each
+    if (tid == 0) {
         calc_trans(t, sh.M, sh.L, sh.tv);
-        build_node_index(t, T, sh.ni);
+    } else if (tid == 64) {
         sh.mainlen = emit_mainlen(t, sh.M);
-        gp(d.roots)[b] = r;
+        gp(d.roots)[b] = sh.root;
+    } else if (tid == 128) {
         int order[MAXN];
         postorder(t, order);
         for (int i = 0; i < N; i++) gp(d.order)[(size_t)b * N + i] = order[i];
-        if (b > 0) {
-            int pr = 0;
-            for (int i = 0; i < N; i++) if (sh.ppar[i] == -1) pr = i;
-            sh.proot = pr;
-            pt.root = pr;
-            count_lineages(pt, T, sh.PL);
-            build_node_index(pt, T, sh.pni);
-            switch_ctx(pt, sh.M, sh.PL, spr, sh.cx);
-        }
+    } else if (tid == 192 && b > 0) {
+        switch_ctx(pt, sh.M, sh.PL, spr, sh.cx);
     }
     __syncthreads();
     t.root = sh.root;
