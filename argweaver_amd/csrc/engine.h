@@ -38,7 +38,7 @@ struct DevInst {
     // prep outputs in state order (pruning and traceback lookups)
     int *snode, *stime;       // [Stot]
     double *eterm;            // [Stot*6]
-    int *order, *roots;       // [B*N] postorder, [B]
+    int *order, *roots;       // [B*N] node levels: height | depth << 16, [B] roots
     double *nmut;             // [B*N*2]
     double *tm, *tm2s;        // [B*(T-1)^2] P(a->b) other branch, [B*TC] same branch
     int *srcs;                // [2B] recombination / recoalescence source of the switch into b

@@ -215,6 +215,7 @@ struct PrepShared {
     int recombsrc, recoalsrc;
     int node3, rparent;
     int scan[MAXN];
+    int hgt[MAXN], dep[MAXN];
 };
 
 // Lineage counts (count_lineages) with one thread per node and integer LDS
@@ -330,6 +331,41 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         sh.L.nb[T - 1] = 1;
         sh.PL.nb[T - 1] = 1;
     }
+    // node levels for the level-synchronous pruning of the emissions:
+    // height (leaves 0) and depth (root 0) as fixed points of parallel
+    // relaxations (monotone, so reading a neighbour's value mid-update is
+    // harmless), one round per level
+    for (int i = tid; i < N; i += NT) {
+        sh.hgt[i] = 0;
+        sh.dep[i] = 0;
+    }
+    __syncthreads();
+    for (int it = 0; it <= N; it++) {
+        int ch = 0;
+        for (int j = tid; j < N; j += NT) {
+            if (sh.ch0[j] == -1) continue;
+            const int h = 1 + max(sh.hgt[sh.ch0[j]], sh.hgt[sh.ch1[j]]);
+            if (h != sh.hgt[j]) {
+                sh.hgt[j] = h;
+                ch = 1;
+            }
+        }
+        if (!__syncthreads_or(ch)) break;
+    }
+    for (int it = 0; it <= N; it++) {
+        int ch = 0;
+        for (int j = tid; j < N; j += NT) {
+            const int p = sh.par[j];
+            if (p == -1) continue;
+            const int dd = sh.dep[p] + 1;
+            if (dd != sh.dep[j]) {
+                sh.dep[j] = dd;
+                ch = 1;
+            }
+        }
+        if (!__syncthreads_or(ch)) break;
+    }
+    for (int i = tid; i < N; i += NT) gp(d.order)[(size_t)b * N + i] = sh.hgt[i] | (sh.dep[i] << 16);
     __syncthreads();
     // the remaining sequential pieces, one wave each
     if (tid == 0) {
@@ -337,10 +373,6 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     } else if (tid == 64) {
         sh.mainlen = emit_mainlen(t, sh.M);
         gp(d.roots)[b] = sh.root;
-    } else if (tid == 128) {
-        int order[MAXN];
-        postorder(t, order);
-        for (int i = 0; i < N; i++) gp(d.order)[(size_t)b * N + i] = order[i];
     } else if (tid == 192 && b > 0) {
         switch_ctx(pt, sh.M, sh.PL, spr, sh.cx);
     }
@@ -513,10 +545,79 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
 //===========================================================================
 // emissions of variant columns, written as lane-run rows
 
+// Felsenstein pruning (pruning) by one wave, level by level: the inner
+// partials of all nodes of one height at once (children first), then the
+// outer partials of all nodes of one depth (parents first).  Every node's
+// arithmetic is the sequential version's, so the results are identical;
+// one wave's LDS operations run in order, so a level sees the one before.
+__device__ inline void pruning_wave(const TreeView &t, const int *lvl, const char *bases,
+                                    const double *muts, const double *nomuts, double *inner,
+                                    double *outer)
+{
+    const int lane = threadIdx.x;
+    int hmax = 0, dmax = 0;
+    for (int j = lane; j < t.n; j += 64) {
+        hmax = max(hmax, lvl[j] & 0xffff);
+        dmax = max(dmax, lvl[j] >> 16);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        hmax = max(hmax, __shfl_xor(hmax, off));
+        dmax = max(dmax, __shfl_xor(dmax, off));
+    }
+    for (int h = 0; h <= hmax; h++) {
+        for (int j = lane; j < t.n; j += 64) {
+            if ((lvl[j] & 0xffff) != h) continue;
+            if (t.c0[j] == -1) {
+                leaf_partial(bases[j], inner + 4 * j);
+                continue;
+            }
+            const int c1 = t.c0[j], c2 = t.c1[j];
+            for (int a = 0; a < 4; a++) {
+                double p1 = 0.0, p2 = 0.0;
+                for (int b = 0; b < 4; b++) {
+                    if (a == b) {
+                        p1 += inner[4 * c1 + b] * nomuts[c1];
+                        p2 += inner[4 * c2 + b] * nomuts[c2];
+                    } else {
+                        p1 += inner[4 * c1 + b] * muts[c1];
+                        p2 += inner[4 * c2 + b] * muts[c2];
+                    }
+                }
+                inner[4 * j + a] = p1 * p2;
+            }
+        }
+        __syncthreads();
+    }
+    for (int dd = 0; dd <= dmax; dd++) {
+        for (int j = lane; j < t.n; j += 64) {
+            if ((lvl[j] >> 16) != dd) continue;
+            if (j == t.root) {
+                for (int a = 0; a < 4; a++) outer[4 * j + a] = 1.0;
+                continue;
+            }
+            const int sib = t.sibling(j);
+            const int parent = t.parent[j];
+            for (int a = 0; a < 4; a++) {
+                double p1 = 0.0, p2 = 0.0;
+                for (int b = 0; b < 4; b++) {
+                    const double ms = (a == b) ? nomuts[sib] : muts[sib];
+                    p1 += inner[4 * sib + b] * ms;
+                    if (parent != t.root) {
+                        const double mp = (a == b) ? nomuts[parent] : muts[parent];
+                        p2 += outer[4 * parent + b] * mp;
+                    }
+                }
+                outer[4 * j + a] = (parent != t.root) ? p1 * p2 : p1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D,
                                                    const int *__restrict__ voff, int ninst)
 {
-    __shared__ int par[MAXN], ch0[MAXN], ch1[MAXN], ag[MAXN], order[MAXN];
+    __shared__ int par[MAXN], ch0[MAXN], ch1[MAXN], ag[MAXN], lvl[MAXN];
     __shared__ char bases[MAXN];
     __shared__ double muts[MAXN], nomuts[MAXN];
     __shared__ double inner[4 * MAXN], outer[4 * MAXN];
@@ -535,17 +636,17 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
         ch0[i] = gp(d.c0)[x];
         ch1[i] = gp(d.c1)[x];
         ag[i] = gp(d.age)[x];
-        order[i] = gp(d.order)[x];
+        lvl[i] = gp(d.order)[x];
         muts[i] = gp(d.nmut)[2 * x];
         nomuts[i] = gp(d.nmut)[2 * x + 1];
         if (i < nleaves) bases[i] = gp(d.seq)[(size_t)i * d.L + col];
     }
     __syncthreads();
     const int root = gp(d.roots)[b];
-    if (lane == 0) {
+    {
         TreeView t{N, root, par, ch0, ch1, ag};
-        pruning(t, order, bases, muts, nomuts, inner, outer);
-        leaf_partial(gp(d.seq)[(size_t)nleaves * d.L + col], in2);
+        pruning_wave(t, lvl, bases, muts, nomuts, inner, outer);
+        if (lane == 0) leaf_partial(gp(d.seq)[(size_t)nleaves * d.L + col], in2);
     }
     __syncthreads();
     const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
