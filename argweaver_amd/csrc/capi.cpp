@@ -735,6 +735,12 @@ void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int m
     o.max_batch = max_batch;
     o.run.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
     o.run.tile_cols = tile_cols;
+    // FP32 requests longer than 50000 columns run as forward tiles
+    if (precision == 1) {
+        o.run.fw_tile = 50000;
+        o.run.fw_warm = 20000;
+        o.run.fw_tol = 1e-4;
+    }
     return new Server(o);
 }
 

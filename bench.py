@@ -127,9 +127,12 @@ def plain_fp64(A, insts, model, budget_s, log):
         b = A.Batch()
         b.add(it["trees"], it["seqs"], times, pops, rho, mu, it["k"] - 1, it["u"])
         b.upload()
-        b.run(0, 0)   # warm (allocations)
+        # the traceback tiling of a drop-in call (run_instance): 4096-column
+        # tiles for instances of 16384 columns or more
+        tile = 4096 if it["L"] >= 16384 else 0
+        b.run(0, tile)   # warm (allocations)
         t0 = time.perf_counter()
-        b.run(0, 0)
+        b.run(0, tile)
         dt = time.perf_counter() - t0
         W += b.W(0)
         t_used += dt
@@ -142,15 +145,16 @@ def plain_fp64(A, insts, model, budget_s, log):
                 seconds=t_used)
 
 
-def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, max_batch=64):
+def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, max_batch=64,
+            precision=1):
     """Poisson arrivals at `rate` requests/s of instances drawn from the
-    workload, served with continuous batching (FP32); per-request latency =
-    finish - arrival on the server clock."""
+    workload, served with continuous batching (FP32 or FP64); per-request
+    latency = finish - arrival on the server clock."""
     import threading
     rng = np.random.RandomState(seed)
     picks = rng.randint(0, len(insts), size=nreq)
     gaps = rng.exponential(1.0 / rate, size=nreq)
-    srv = A.Server(hbm_budget_gb=budget_gb, precision=1, tile_cols=tile, max_batch=max_batch)
+    srv = A.Server(hbm_budget_gb=budget_gb, precision=precision, tile_cols=tile, max_batch=max_batch)
     ids = [None] * nreq
 
     def producer():
@@ -171,7 +175,8 @@ def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, m
     for j, i in enumerate(ids):
         res.append(srv.wait(i, timeout=120))
         if (j + 1) % 64 == 0:
-            log("serving %.0f req/s: %d/%d done at %.1f s" % (rate, j + 1, nreq, srv.now() - t0))
+            log("serving %.0f req/s (%s): %d/%d done at %.1f s"
+                % (rate, "fp32" if precision else "fp64", j + 1, nreq, srv.now() - t0))
     wall = srv.now() - t0
     srv.close()
     lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
@@ -180,20 +185,28 @@ def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, m
     out = dict(rate=rate, requests=nreq, p50_ms=float(np.percentile(lat, 50)),
                p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
                throughput=W / wall, unit="column-states/s", wall_s=wall, batches=nb,
-               max_batch=max_batch)
-    log("serving %.0f req/s: p50 %.1f ms, p99 %.1f ms, %.3g column-states/s, %d batches"
-        % (rate, out["p50_ms"], out["p99_ms"], out["throughput"], nb))
+               max_batch=max_batch, precision="fp32" if precision else "fp64")
+    log("serving %.0f req/s (%s): p50 %.1f ms, p99 %.1f ms, %.3g column-states/s, %d batches"
+        % (rate, out["precision"], out["p50_ms"], out["p99_ms"], out["throughput"], nb))
     return out
+
+
+def mcmc_model(seed):
+    """Config-5 model: 20 time points, N(t) piecewise constant over the 20
+    intervals with values log-uniform in 5e3..5e4 (seeded)."""
+    rng = np.random.RandomState(seed + 11)
+    pops = np.exp(rng.uniform(np.log(5e3), np.log(5e4), size=20))
+    return pops, 1.6e-8 * 10, 1.8e-8 * 10
 
 
 def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
     """Config-5 sample: the MCMC step of resample_arg on a k=100, 5 Mb
     (500k columns at c=10) ARG under a piecewise-constant N(t): remove a
     leaf, re-thread it through the drop-in arghmm_sample_thread (forward,
-    traceback, recombinations and ARG surgery).  Times nrethreads steps."""
+    traceback, recombinations and ARG surgery).  Times nrethreads steps and
+    records the number of local trees (recombinations + 1) after each."""
     times = np.array(A.get_time_points(20, 200e3), np.float64)
-    pops = np.array([1e4] * 6 + [2e3] * 4 + [1e4] * 10, np.float64)   # a bottleneck
-    rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
+    pops, rho, mu = mcmc_model(seed)
     t0 = time.time()
     tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed)
     log("mcmc workload: k=%d L=%d simulated in %.1f s" % (k, L, time.time() - t0))
@@ -207,12 +220,94 @@ def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
         W, _ = A.num_column_states(tr, 20)
         A.sample_thread(tr, seqs, times, pops, rho, mu)
         dt = time.perf_counter() - t0
-        out.append((dt, W))
-        log("mcmc re-thread %d: W=%.3g, %.2f s" % (j, W, dt))
+        out.append((dt, W, int(A.get_local_trees_ntrees(tr))))
+        log("mcmc re-thread %d: W=%.3g, %.2f s, %d local trees" % (j, W, dt, out[-1][2]))
     # the first call includes one-time allocations
-    dt, W = out[-1]
-    return dict(k=k, columns=L, popsizes="piecewise 1e4/2e3/1e4", rethread_s=dt,
-                column_states=W, value=W / dt, unit="column-states/s")
+    dt, W, _ = out[-1]
+    return dict(k=k, columns=L, popsizes="piecewise constant, 20 intervals, log-uniform 5e3..5e4",
+                rethread_s=dt, column_states=W, value=W / dt, unit="column-states/s",
+                local_trees_trace=[o[2] for o in out])
+
+
+def mcmc_cpu_compare(A, seed, log, k=100, L=500000):
+    """The reference chain beside the GPU one (CPU-baseline leg): the same
+    config-5 ARG, one re-threading step from the same srand seed in the
+    reference library and in the drop-in; reports whether the two ARGs are
+    identical, their local-tree counts and the two times."""
+    import oracle_bind as O
+    if not O.ref_available():
+        return None
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
+    pops, rho, mu = mcmc_model(seed)
+    tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed)
+    rt = O.ref_new_trees(*A.trees_arrays(tr))
+    A.remove_thread(tr, k - 1)
+    O.ref_remove_thread(rt, k - 1)
+    O.srand(seed + 3)
+    t0 = time.perf_counter()
+    A.sample_thread(tr, seqs, times, pops, rho, mu)
+    t_gpu = time.perf_counter() - t0
+    O.srand(seed + 3)
+    t0 = time.perf_counter()
+    O.ref_sample_thread(rt, seqs, times, pops, rho, mu)
+    t_cpu = time.perf_counter() - t0
+    mine, ref = A.trees_arrays(tr), O.ref_trees_arrays(rt)
+    same = all(np.array_equal(a, b) for a, b in zip(mine, ref))
+    out = dict(identical_arg=bool(same), local_trees_gpu=int(len(mine[3])),
+               local_trees_cpu=int(len(ref[3])), seconds_gpu=t_gpu, seconds_cpu=t_cpu,
+               cpu_cores=1)
+    O.ref_delete(rt)
+    log("mcmc step vs reference chain: identical ARG %s, local trees %d / %d, "
+        "%.2f s GPU, %.2f s CPU" % (same, out["local_trees_gpu"], out["local_trees_cpu"],
+                                    t_gpu, t_cpu))
+    return out
+
+
+def single_large(A, seed, log, k=50, L=100000):
+    """Config-2 instance (k=50, 1 Mb at c=10): the FP64 drop-in schedule
+    against the FP32 one (forward tiles), with the FP32 table's error
+    against FP64 on normalised columns and the log-likelihood error."""
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
+    pops = np.full(20, 1e4)
+    rho, mu = 1.6e-8 * 10, 1.8e-8 * 10
+    tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed + 5)
+    A.remove_thread(tr, k - 1)
+    u = np.random.RandomState(seed + 5).uniform(size=L)
+    b = A.Batch()
+    b.add(tr, seqs, times, pops, rho, mu, k - 1, u)
+    b.upload()
+    res = {}
+    for prec, kw in ((0, {}), (1, dict(fw_tile=50000, fw_warm=20000, fw_tol=1e-4))):
+        b.run(prec, 4096, **kw)
+        t0 = time.perf_counter()
+        st = b.run(prec, 4096, **kw)
+        res[prec] = dict(seconds=time.perf_counter() - t0, lnl=b.lnl(0), alpha=b.alpha(0),
+                         W=st["W"])
+    W = res[0]["W"]
+    # normalised columns: block-major rows of S entries per column
+    a64, a32 = res[0]["alpha"], res[1]["alpha"]
+    nst = A.get_nstates(tr, 20, L)
+    off = np.concatenate([[0], np.cumsum(nst)])
+    err_abs = 0.0
+    err_rel = 0.0
+    for c in range(0, L, 97):   # a sample of columns
+        x = a64[off[c]:off[c + 1]]
+        y = a32[off[c]:off[c + 1]]
+        x = x / x.sum()
+        y = y / y.sum()
+        err_abs = max(err_abs, float(np.abs(x - y).max()))
+        big = x > 1e-3 * x.max()
+        err_rel = max(err_rel, float((np.abs(x - y)[big] / x[big]).max()))
+    out = dict(k=k, columns=L, column_states=W, fp64_seconds=res[0]["seconds"],
+               fp32_seconds=res[1]["seconds"], fp64_value=W / res[0]["seconds"],
+               fp32_value=W / res[1]["seconds"],
+               rel_dlogL=abs(res[1]["lnl"] - res[0]["lnl"]) / abs(res[0]["lnl"]),
+               alpha_max_abs_err_normalised=err_abs, alpha_max_rel_err_large_entries=err_rel,
+               sampled_columns=len(range(0, L, 97)))
+    log("single large k=%d L=%d: fp64 %.3f s, fp32 %.3f s, |dlogL|/|logL| %.2e, alpha err abs %.2e"
+        " rel %.2e" % (k, L, out["fp64_seconds"], out["fp32_seconds"], out["rel_dlogL"], err_abs,
+                       err_rel))
+    return out
 
 
 def main():
@@ -237,6 +332,9 @@ def main():
                     help="Poisson request rates (req/s), comma-separated")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
+    ap.add_argument("--no-large", action="store_true", help="skip the config-2 single instance")
+    ap.add_argument("--serve-fp64-rate", type=float, default=200.0,
+                    help="one FP64 serving run at this rate (0: none)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_all_batched256_traffic.md"),
@@ -322,6 +420,13 @@ def main():
         extra["serving"] = [serving(A, insts, model, float(r), args.serve_requests, args.seed + j,
                                     log, tile=args.tile)
                             for j, r in enumerate(args.serve_rates.split(","))]
+        if args.serve_fp64_rate > 0:
+            extra["serving"].append(serving(A, insts, model, args.serve_fp64_rate,
+                                            args.serve_requests, args.seed + 97, log,
+                                            tile=args.tile, precision=0))
+
+    if not args.no_large:
+        extra["single_large"] = single_large(A, args.seed, log)
 
     if not args.no_mcmc:
         extra["mcmc_rethread"] = mcmc_rethread(A, args.seed, log)
@@ -344,6 +449,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(A, insts, model, args.cpu_budget)
         log("cpu baseline: %.3g column-states/s (%s)" % (cpu["value"], cpu["kind"]))
+        if not args.no_mcmc:
+            cmp = mcmc_cpu_compare(A, args.seed, log)
+            if cmp is not None:
+                extra.setdefault("mcmc_rethread", {})["vs_reference_chain"] = cmp
 
     out = {
         "metric": "threaded columns*states/s (fwd+traceback) FP32-batched vs FP64; p99 latency",

@@ -203,7 +203,9 @@ LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
  * times3 = {submitted, started, finished} in seconds on the server clock.
  * It returns 0 on success, -2 on time-out (the request stays pending, wait
  * again), -1 for an unknown id, else the failed batch's status.  A
- * request's result does not depend on the batch it ran in. */
+ * request's result does not depend on the batch it ran in.  FP32 servers
+ * run requests longer than 50000 columns as forward tiles (as
+ * awamd_batch_run_ex with fw_tile 50000, fw_warm 20000, fw_tol 1e-4). */
 void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch);
 void awamd_server_free(void *server);
 int awamd_server_submit(void *server, LocalTrees *trees, const double *times, int ntimes,
