@@ -2342,6 +2342,13 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
     // long instances trace back in tiles (the thread is the same as with one
     // tile: fix-up passes re-sample until the chains coalesce)
     opt.tile_cols = (tile_cols == 0 && hi.ft.seqlen >= 16384) ? 4096 : tile_cols;
+    // FP32 forwards of long instances run as tiles (FP64 keeps the
+    // sequential forward: its contract is the reference's table)
+    if (prec == Precision::FP32) {
+        opt.fw_tile = 50000;
+        opt.fw_warm = 20000;
+        opt.fw_tol = 1e-4;
+    }
     if (bt.run(opt, stats)) return 1;
     if (path && u && bt.copy_path(0, path)) return 1;
     if (alpha && bt.copy_alpha(0, alpha)) return 1;

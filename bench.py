@@ -229,6 +229,32 @@ def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
                 local_trees_trace=[o[2] for o in out])
 
 
+def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
+    """Config-5 in FP32: re-thread leaves k-1, k-2, .. in turn (remove, FP32
+    forward with forward tiles and traceback, recombinations, add), with the
+    log-likelihood and local-tree traces of the chain."""
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
+    pops, rho, mu = mcmc_model(seed)
+    tr, seqs = A.simulate(k, L, times, pops, rho, mu, seed)
+    rng = np.random.RandomState(seed + 21)
+    A.set_random_seed(seed + 22)
+    steps = []
+    for j in range(nsteps):
+        leaf = k - 1 - j
+        t0 = time.perf_counter()
+        A.remove_thread(tr, leaf)
+        r = A.thread_forward(tr, seqs, times, pops, rho, mu, leaf, rng.uniform(size=L),
+                             precision=1, want_alpha=False)
+        pos, node, tm = A.sample_recombinations(tr, times, pops, rho, mu, r["path"])
+        A.add_thread(tr, len(times), r["path"], leaf, pos, node, tm)
+        dt = time.perf_counter() - t0
+        steps.append(dict(leaf=leaf, seconds=dt, lnl=r["lnl"], recombinations=int(len(pos)),
+                          local_trees=int(A.get_local_trees_ntrees(tr))))
+        log("mcmc fp32 re-thread leaf %d: %.2f s, logL %.6g, %d recombinations, %d local trees"
+            % (leaf, dt, r["lnl"], len(pos), steps[-1]["local_trees"]))
+    return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])))
+
+
 def mcmc_cpu_compare(A, seed, log, k=100, L=500000):
     """The reference chain beside the GPU one (CPU-baseline leg): the same
     config-5 ARG, one re-threading step from the same srand seed in the
@@ -430,6 +456,7 @@ def main():
 
     if not args.no_mcmc:
         extra["mcmc_rethread"] = mcmc_rethread(A, args.seed, log)
+        extra["mcmc_rethread"]["fp32_chain"] = mcmc_rethread_fp32(A, args.seed, log)
 
     # dominant kernel: the FP32 forward's main launch (k_fw_all<float, 19,
     # false>, one per step; re-run tiles are a separate symbol); algorithmic
