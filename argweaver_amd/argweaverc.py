@@ -128,6 +128,8 @@ awamd_batch_run_ex = _export(
     "awamd_batch_run_ex", c_int,
     [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p])
 awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
+awamd_batch_tune_forward = _export(
+    "awamd_batch_tune_forward", c_int, [c_void_p, c_int_p, c_int_p, c_int, c_double, c_double_p])
 awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
 awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_double_p])
@@ -414,6 +416,19 @@ class Batch(object):
                               int(fw_tile), int(fw_warm), float(fw_tol), _dptr(st)) != 0:
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
+
+    def tune_forward(self, candidates=((0, 0), (40000, 15000), (50000, 20000), (60000, 25000),
+                                       (80000, 25000)), tol=1e-4):
+        """Autotune the FP32 forward tiles on this batch; returns the best
+        (tile, warm-up) pair and {pair: forward ms}."""
+        tiles = np.ascontiguousarray([c[0] for c in candidates], np.int32)
+        warms = np.ascontiguousarray([c[1] for c in candidates], np.int32)
+        ms = np.zeros(len(candidates), np.float64)
+        best = awamd_batch_tune_forward(self.ptr, _iptr(tiles), _iptr(warms), len(candidates),
+                                        float(tol), _dptr(ms))
+        if best < 0:
+            raise RuntimeError("awamd_batch_tune_forward failed")
+        return tuple(candidates[best]), {tuple(c): float(m) for c, m in zip(candidates, ms)}
 
     def tune(self, precision=1, candidates=(0, 500, 1000, 2000, 4000, 8000)):
         """Autotune the traceback tile length on this batch; returns

@@ -656,6 +656,12 @@ int awamd_batch_tune(void *b, int precision, const int *cand, int ncand, double 
         precision == 1 ? Precision::FP32 : Precision::FP64, cand, ncand, ms_out);
 }
 
+int awamd_batch_tune_forward(void *b, const int *tiles, const int *warms, int ncand, double tol,
+                             double *ms_out)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.tune_forward(tiles, warms, ncand, tol, ms_out);
+}
+
 int awamd_batch_path(void *b, int i, int *path)
 {
     return reinterpret_cast<AwBatch *>(b)->batch.copy_path(i, path);

@@ -155,6 +155,11 @@ public:
     // candidate in ms_out when given), -1 on error.  Leaves the batch with
     // the path of the last candidate.
     int tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out = nullptr);
+    // FP32 forward-tile autotuner: times the forward (with its checks and
+    // re-runs) for each (tile, warm-up) candidate and returns the index of
+    // the fastest (-1 on error)
+    int tune_forward(const int *tiles, const int *warms, int ncand, double tol,
+                     double *ms_out = nullptr);
     int copy_path(int i, int *path) const;
     int copy_alpha(int i, double *alpha) const;   // block-major FP64 view of the table
     int copy_norms(int i, double *norms) const;

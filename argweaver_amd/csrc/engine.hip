@@ -1942,6 +1942,28 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
     return best;
 }
 
+int Batch::tune_forward(const int *tiles, const int *warms, int ncand, double tol, double *ms_out)
+{
+    int best = -1;
+    double best_ms = 0;
+    for (int i = 0; i < ncand; i++) {
+        BatchOptions opt;
+        opt.prec = Precision::FP32;
+        opt.traceback = false;
+        opt.fw_tile = tiles[i];
+        opt.fw_warm = warms[i] > 0 ? warms[i] : opt.fw_warm;
+        if (tol > 0) opt.fw_tol = tol;
+        RunStats rs;
+        if (run(opt, &rs)) return -1;
+        if (ms_out) ms_out[i] = rs.ms_forward;
+        if (best < 0 || rs.ms_forward < best_ms) {
+            best_ms = rs.ms_forward;
+            best = i;
+        }
+    }
+    return best;
+}
+
 // The forward of every instance.  FP32 runs may split long instances into
 // speculative tiles (BatchOptions::fw_tile); each tile's start is then
 // checked against the boundary column of the tile before, and tiles that

@@ -345,8 +345,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tile", type=int, default=-1,
                     help="traceback tile length in columns (-1: autotune, 0: one tile per instance)")
-    ap.add_argument("--fw-tile", type=int, default=50000,
-                    help="FP32 forward tile length in columns (0: one tile per instance)")
+    ap.add_argument("--fw-tile", type=int, default=-1,
+                    help="FP32 forward tile length in columns (0: one tile per instance, "
+                         "-1: autotune with the warm-up)")
     ap.add_argument("--fw-warm", type=int, default=20000, help="warm-up columns of a forward tile")
     ap.add_argument("--fw-tol", type=float, default=1e-4,
                     help="relative tolerance of a forward tile's start check")
@@ -397,6 +398,11 @@ def main():
         it["W"] = batch.W(i)
     log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
 
+    if args.fw_tile < 0:
+        (args.fw_tile, args.fw_warm), fms = batch.tune_forward(tol=args.fw_tol)
+        log("autotuned forward tiles: %d columns, warm-up %d (%s)"
+            % (args.fw_tile, args.fw_warm,
+               ", ".join("%d/%d: %.1f ms" % (t, w, v) for (t, w), v in fms.items())))
     if args.tile < 0:
         tile, tms = batch.tune(1)
         log("autotuned traceback tile: %d (%s)" % (tile, ", ".join("%d: %.1f ms" % kv for kv in tms.items())))

@@ -179,6 +179,12 @@ int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback,
  * for each candidate tile length (0: one tile per instance) and returns the
  * fastest (-1 on error); ms_out[ncand] receives the timings when given. */
 int awamd_batch_tune(void *batch, int precision, const int *cand, int ncand, double *ms_out);
+/* FP32 forward-tile autotuner: times the forward (checks and re-runs
+ * included) for each candidate (tiles[i] columns, warms[i] warm-up columns;
+ * tiles[i] = 0: one tile per instance) at tolerance tol and returns the
+ * index of the fastest (-1 on error); ms_out[ncand] receives the timings. */
+int awamd_batch_tune_forward(void *batch, const int *tiles, const int *warms, int ncand,
+                             double tol, double *ms_out);
 int awamd_batch_path(void *batch, int i, int *path);
 int awamd_batch_alpha(void *batch, int i, double *alpha);
 int awamd_batch_norms(void *batch, int i, double *norms);

@@ -160,3 +160,13 @@ def test_tile_autotuner(aw, mix):
     b.run(1, best)
     for i in range(len(mix)):
         assert np.array_equal(b.path(i), ref[i]), i
+
+
+def test_forward_tile_autotuner(aw, mix):
+    """The forward-tile tuner times every candidate and returns one of them."""
+    print("forward tile autotuner", flush=True)
+    b = _batch(aw, mix)
+    cands = ((0, 0), (900, 600), (1500, 600))
+    best, ms = b.tune_forward(candidates=cands)
+    assert best in cands
+    assert set(ms) == set(cands) and all(v > 0 for v in ms.values())
