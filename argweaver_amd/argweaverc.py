@@ -418,7 +418,7 @@ class Batch(object):
         return dict(zip(self.STAT_NAMES, st.tolist()))
 
     def tune_forward(self, candidates=((0, 0), (40000, 15000), (50000, 20000), (60000, 25000),
-                                       (80000, 25000)), tol=1e-4):
+                                       (80000, 25000), (100000, 40000)), tol=1e-4):
         """Autotune the FP32 forward tiles on this batch; returns the best
         (tile, warm-up) pair and {pair: forward ms}."""
         tiles = np.ascontiguousarray([c[0] for c in candidates], np.int32)

@@ -1342,9 +1342,18 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
     TbLane<R> tl;
     tb_load_block<Real, R>(d, bc, tl, tab);
     // window of 64 columns w0, w0-1, .., w0-63: lane j holds column w0-j
+    // (its uniform, its stored path in mode 1, and the sampled state, which
+    // is stored once per window: per-column stores would make every later
+    // load wait for them, vmcnt being in order)
     int w0 = hi;
     double uw = ug[min(L - 1 - (w0 - lane), L - 1)];
     int pw = (mode == 1) ? pg[max(w0 - lane, 0)] : -1;
+    int pnew = 0;
+    // store the sampled states of the window's columns w0 .. clo
+    auto flush = [&](int clo) {
+        const int c = w0 - lane;
+        if (c >= clo && c <= min(w0, whi)) path[c] = pnew;
+    };
     Real nxt[BT][R];
 #pragma unroll
     for (int k = 0; k < BT; k++) row_load<Real, R>(alpha, max(hi - k, lo), nxt[k]);
@@ -1360,6 +1369,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
         const Real *rb = rows + buf * BT * RW;
         for (int col = top; col >= bot; col--) {
             if (w0 - col >= 64) {
+                flush(col + 1);
                 w0 = col;
                 uw = ug[min(L - 1 - (w0 - lane), L - 1)];
                 if (mode == 1) pw = pg[max(w0 - lane, 0)];
@@ -1403,8 +1413,11 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
             }
             knext = lane_sample<R>(pre, tl.ti, S, uval, kinfo);
             kblk = bc;
-            if (mode == 1 && oldp == knext) return col;
-            if (lane == 0 && col <= whi) path[col] = knext;
+            if (mode == 1 && oldp == knext) {
+                flush(col + 1);
+                return col;
+            }
+            pnew = (lane == w0 - col) ? knext : pnew;
             if (col - 1 < bc_start && col - 1 >= lo) {
                 bc--;
                 bc_start = cp(d.bstart)[bc];
@@ -1420,6 +1433,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
             for (int r = 0; r < R; r++) wb[k * RW + r * 64 + lane] = nxt[k][r];
         lds_order();
     }
+    flush(lo);
     return -1;
 }
 
@@ -1978,10 +1992,16 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
     std::vector<FwTile> tl;
     std::vector<int> next;   // tile index of the following tile of its instance, -1 for none
     int nslot = 0;
-    const int C = (fp32 && opt.fw_tile > 0) ? opt.fw_tile : 0;
+    const int C0 = (fp32 && opt.fw_tile > 0) ? opt.fw_tile : 0;
     const int W = std::max(opt.fw_warm, 1);
     for (int i : order) {
         const int L = m.h[i].L;
+        // instances whose columns cost more than R = 8's get proportionally
+        // shorter tiles, so that the longest tiles of every R take about
+        // the same time (measured FP32 column costs, us)
+        const double cost8 = 0.94;
+        const double costR = m.h[i].R <= 8 ? cost8 : m.h[i].R == 12 ? 1.20 : 1.50;
+        const int C = C0 ? std::max(W + W / 4, (int)(C0 * cost8 / costR)) : 0;
         if (!C || L <= C || C <= W) {
             tl.push_back(FwTile{i, 0, 0, L, 0, -1, -1});
             next.push_back(-1);
