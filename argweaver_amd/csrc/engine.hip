@@ -1208,6 +1208,7 @@ struct Chunk {
 template <int R>
 struct TbLane {
     int ti[R];
+    int node[R];   // node of the state (info_node)
     int off1[R];   // a * (T-1): other-branch entry P(a -> b) at off1 + b
     int off2[R];   // same-branch entry P(a -> b) at off2 + b
     int dt[R];     // deterministic switch target into the next block
@@ -1259,6 +1260,7 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
         // padding slots carry time bin 31: clamp so their (zero-weighted)
         // table reads stay inside the tables
         const int a = min(tl.ti[r] & INFO_TIME, T1 - 1), c = info_age(tl.ti[r]);
+        tl.node[r] = info_node(tl.ti[r]);
         tl.off1[r] = a * T1;
         tl.off2[r] = TT + tc_base(c, T1) + (a - c) * (T1 - c) - c;
     }
@@ -1282,9 +1284,10 @@ __device__ __forceinline__ int lane_sample(const double (&pre)[R], const int (&t
     const unsigned long long m = __ballot(has && excl + loc >= pick);
     const int owner = m ? (int)__ffsll((long long)m) - 1 : (S - 1) / R;
     const int nloc = min(R, S - lane * R);
+    const double thr = pick - excl;
     int cnt = 0;
 #pragma unroll
-    for (int r = 0; r < R; r++) cnt += (excl + pre[r] < pick) ? 1 : 0;
+    for (int r = 0; r < R; r++) cnt += (pre[r] < thr) ? 1 : 0;
     const int rsel = m ? min(cnt, nloc - 1) : nloc - 1;
     // select ti[rsel] with masks (a compare-and-branch form becomes a
     // dynamically indexed scratch array)
@@ -1382,7 +1385,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
             Real rv[R];
 #pragma unroll
             for (int r = 0; r < R; r++) rv[r] = row[r * 64 + lane];
-            double tr[R];
+            Real tr[R];
             if (bc != kblk) {
                 // col is the last column of block bc: switch into kblk = bc+1
                 // (TransMatrixSwitch::get, trans.h:201-209)
@@ -1393,22 +1396,20 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int s = lane * R + r;
-                    tr[r] = (s == cs) ? crk : (s == rs) ? rrk : (tl.dt[r] == knext ? tl.dp[r] : 0.0);
+                    tr[r] = (Real)((s == cs) ? crk : (s == rs) ? rrk : (tl.dt[r] == knext ? tl.dp[r] : 0.0));
                 }
             } else {
                 const int nk = info_node(kinfo), bk = kinfo & INFO_TIME;
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const bool same = info_node(tl.ti[r]) == nk;
-                    tr[r] = (double)tab[(same ? tl.off2[r] : tl.off1[r]) + bk];
-                }
+                for (int r = 0; r < R; r++) tr[r] = tab[(tl.node[r] == nk ? tl.off2[r] : tl.off1[r]) + bk];
             }
+            // padding slots hold 0 in the table, so their weights are 0; the
+            // product is in the table's precision, the cumulative sums FP64
             double pre[R];
             double acc = 0.0;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const double vm = (double)((tl.ti[r] >> 10) & 1);   // INFO_VALID
-                acc += vm * ((double)rv[r] * tr[r]);
+                acc += (double)(rv[r] * tr[r]);
                 pre[r] = acc;
             }
             knext = lane_sample<R>(pre, tl.ti, S, uval, kinfo);
@@ -1456,7 +1457,7 @@ __device__ __forceinline__ int tb_column(const DevInst &d, int col, double uval,
     double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        acc += (double)((ti[r] >> 10) & 1) * (double)rv[r];
+        acc += (double)rv[r];   // padding slots hold 0
         pre[r] = acc;
     }
     blk = b;
