@@ -1249,11 +1249,10 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
                 tl.dp[r] = dpn[r * 64 + lane];
             }
         }
+        // entries past the end repeat the last one (same index, same value):
+        // a per-entry condition would become a branch per store
 #pragma unroll
-        for (int k = 0; k < TCP; k++) {
-            const int x = base + k * 64 + lane;
-            if (x < n) tab[x] = (Real)t[k];
-        }
+        for (int k = 0; k < TCP; k++) tab[min(base + k * 64 + lane, n - 1)] = (Real)t[k];
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
