@@ -1297,13 +1297,21 @@ __device__ __forceinline__ int lane_sample(const double (&pre)[R], const int (&t
     return __builtin_amdgcn_readlane(lane * R + rsel, owner);
 }
 
-// forward-table rows staged per traceback batch
+// forward-table rows staged per traceback batch: about TB_ROWREG registers
+// of a lane hold the next batch (fewer rows per batch also mean less LDS per
+// tile, so more tiles fit on a CU)
+constexpr int TB_ROWREG = 64;
 template <class Real, int R>
 constexpr int tb_batch()
 {
-    return (64 / (R * (int)(sizeof(Real) / 4))) > 16 ? 16
-         : (64 / (R * (int)(sizeof(Real) / 4))) < 2 ? 2
-                                                     : (64 / (R * (int)(sizeof(Real) / 4)));
+    return (TB_ROWREG / (R * (int)(sizeof(Real) / 4))) > 16 ? 16
+         : (TB_ROWREG / (R * (int)(sizeof(Real) / 4))) < 2 ? 2
+                                                           : (TB_ROWREG / (R * (int)(sizeof(Real) / 4)));
+}
+// host mirror of tb_batch
+inline int tb_batch_host(int R, size_t rsz)
+{
+    return std::max(2, std::min(16, TB_ROWREG / (R * (int)(rsz / 4))));
 }
 
 template <class Real, int R>
@@ -1856,7 +1864,7 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
         }
         // (a,b) and (c,a,b) tables, then two row batches
         const int T1 = d.T - 1;
-        const int bt = std::max(2, std::min(16, 64 / (d.R * (int)(rsz / 4))));
+        const int bt = tb_batch_host(d.R, rsz);
         sht = std::max(sht, rsz * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
     const int nch = (int)chs.size();
