@@ -216,6 +216,8 @@ struct PrepShared {
     int node3, rparent;
     int scan[MAXN];
     int hgt[MAXN], dep[MAXN];
+    double rates[2 * MAXT + 1], Cs[2 * MAXT + 2], lnBt[MAXT];
+    double treelen;
 };
 
 // Lineage counts (count_lineages) with one thread per node and integer LDS
@@ -367,9 +369,77 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     }
     for (int i = tid; i < N; i += NT) gp(d.order)[(size_t)b * N + i] = sh.hgt[i] | (sh.dep[i] << 16);
     __syncthreads();
+    // TransMatrix vectors (calc_trans) split over the workgroup with the
+    // sequential version's expressions and summation orders: the tree
+    // length in node order, the rate prefix and the lnB log-sum in time
+    // order, the other per-time terms one thread each
+    {
+        const Model &M = sh.M;
+        double *rates = sh.rates + 1, *C = sh.Cs + 2;
+        if (tid == 0) {
+            double len = 0.0;
+            for (int i = 0; i < N; i++) {
+                const int p = t.parent[i];
+                if (p != -1) len += M.times[t.age[p]] - M.times[t.age[i]];
+            }
+            sh.treelen = len;
+        }
+        if (tid == 64) rates[-1] = 0.0;
+        for (int i = tid; i < 2 * T - 1; i += NT)
+            rates[i] = M.csteps[i] * sh.L.nb[i / 2] / (2.0 * M.pop[i / 2]);
+        __syncthreads();
+        if (tid == 64) {
+            C[-2] = 0.0;
+            C[-1] = 0.0;
+            double c = 0.0;
+            for (int b2 = 0; b2 < 2 * T - 1; b2++) {
+                c = c + rates[b2];
+                C[b2] = c;
+            }
+        }
+        __syncthreads();
+        TransVec &tv = sh.tv;
+        const int ri = t.age[t.root];
+        const double root_age = M.times[ri];
+        const double len = sh.treelen;
+        for (int bb = tid; bb < T; bb += NT) {
+            tv.D[bb] = tv.E[bb] = tv.lnB[bb] = tv.lnE2[bb] = tv.lnNegG1[bb] = 0.0;
+            tv.G2[bb] = tv.G3[bb] = tv.lnG4[bb] = tv.norecombs[bb] = 0.0;
+            if (bb >= T - 1) continue;
+            const int nb = sh.L.nb[bb], nr = sh.L.nr[bb], nc = sh.L.nc[bb];
+            double len2 = len + M.times[bb];
+            double len2_b;
+            if (bb > ri) {
+                len2 += M.times[bb] - root_age;
+                len2_b = len2 + M.steps[bb];
+            } else {
+                len2_b = len2 + M.steps[ri];
+            }
+            sh.lnBt[bb] = C[2 * bb - 1] + log(M.steps[bb] * (nb + 1.0) / (nr + 1.0));
+            double e2 = -C[2 * bb - 2] + (bb < T - 2 ? log(1 - exp(-rates[2 * bb] - rates[2 * bb - 1])) : 0.0);
+            tv.lnE2[bb] = e2;
+            tv.lnG4[bb] = e2;
+            tv.lnNegG1[bb] = C[2 * bb - 1] +
+                log(-M.steps[bb] * ((nb / (nr + 1.0 + int(bb < ri))) - (nb + 1.0) / (nr + 1.0)));
+            double q = (bb < T - 2 ? 1.0 - exp(-rates[2 * bb]) : 1.0);
+            tv.G2[bb] = q * M.steps[bb] * (nb + 1.0) / (nr + 1.0);
+            tv.G3[bb] = q * M.steps[bb] * (nb / (nr + 1.0 + int(bb < ri)));
+            tv.D[bb] = (1.0 - exp(-M.rho * len2)) / len2_b;
+            tv.E[bb] = 1.0 / nc;
+            tv.norecombs[bb] = exp(-fmax(M.rho * len2, M.rho));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double lb = sh.lnBt[0];
+            tv.lnB[0] = lb;
+            for (int bb = 1; bb < T - 1; bb++) {
+                lb = logadd(lb, sh.lnBt[bb]);
+                tv.lnB[bb] = lb;
+            }
+        }
+    }
     // the remaining sequential pieces, one wave each
     if (tid == 0) {
-        calc_trans(t, sh.M, sh.L, sh.tv);
     } else if (tid == 64) {
         sh.mainlen = emit_mainlen(t, sh.M);
         gp(d.roots)[b] = sh.root;
