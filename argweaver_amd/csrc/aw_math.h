@@ -409,7 +409,29 @@ __host__ __device__ inline double emit_mainlen(const TreeView &t, const Model &m
     return len;
 }
 
-__host__ __device__ inline void emit_terms(const TreeView &t, const Model &m, double mainlen,
+// the terms that depend on the state's time only: the new leaf's branch
+// (dist[0]) and the invariant-column likelihood, for a state off and on the
+// root branch
+struct EmitTime {
+    double mut0, nomut0, inv, inv_root;
+};
+
+__host__ __device__ inline void emit_time_terms(const TreeView &t, const Model &m, double mainlen,
+                                                int time, EmitTime &et)
+{
+    const double mintime = m.times[1] * .1;
+    const double coal_time = m.times[time];
+    const double d0 = fmax(coal_time - 0.0, mintime);
+    et.mut0 = prob_branch(d0, m.mu, true);
+    et.nomut0 = prob_branch(d0, m.mu, false);
+    const double len = mainlen + fmax(coal_time - 0.0, mintime);
+    et.inv = .25 * exp(-m.mu * fmax(len, mintime));
+    const double len_r = mainlen + fmax(coal_time - 0.0, mintime) +
+                         fmax(coal_time - m.times[t.age[t.root]], mintime);
+    et.inv_root = .25 * exp(-m.mu * fmax(len_r, mintime));
+}
+
+__host__ __device__ inline void emit_terms(const TreeView &t, const Model &m, const EmitTime &et,
                                            int node2, int time, EmitTerms &e)
 {
     const double mintime = m.times[1] * .1;
@@ -418,20 +440,15 @@ __host__ __device__ inline void emit_terms(const TreeView &t, const Model &m, do
     double parent_time = (parent != -1) ? m.times[min(t.age[parent], m.T - 1)] : 0.0;
     double coal_time = m.times[time];
     double dist[3];
-    dist[0] = fmax(coal_time - 0.0, mintime);
     dist[1] = fmax(coal_time - time2, mintime);
     dist[2] = fmax(parent_time - coal_time, mintime);
-    for (int x = 0; x < 3; x++) {
+    e.mut[0] = et.mut0;
+    e.nomut[0] = et.nomut0;
+    for (int x = 1; x < 3; x++) {
         e.mut[x] = prob_branch(dist[x], m.mu, true);
         e.nomut[x] = prob_branch(dist[x], m.mu, false);
     }
-    double len;
-    if (node2 == t.root)
-        len = mainlen + fmax(coal_time - 0.0, mintime) +
-              fmax(coal_time - m.times[t.age[t.root]], mintime);
-    else
-        len = mainlen + fmax(coal_time - 0.0, mintime);
-    e.inv = .25 * exp(-m.mu * fmax(len, mintime));
+    e.inv = (node2 == t.root) ? et.inv_root : et.inv;
 }
 
 // calc_emit for one state given the node's inner/outer partials and the

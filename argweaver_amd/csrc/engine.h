@@ -44,8 +44,8 @@ struct DevInst {
     int *srcs;                // [2B] recombination / recoalescence source of the switch into b
     // prep outputs in the lane-run layout, one row of RW per block
     int *linfo;               // [B*RW] time | flags | age(node) << 11 | node << 16
-    double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef)
-    float *lcoef32;           // [B*NCOEF*RW] the planes in FP32 (FP32 runs)
+    double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef), FP64 runs
+    float *lcoef32;           // [B*NCOEF*RW] the planes in FP32, FP32 runs
     int fp32;                 // the run is FP32
     int *ldet;                // [B*RW] deterministic switch target of block b-1's states
     double *lprior;           // [RW] first-column prior

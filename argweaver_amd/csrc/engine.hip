@@ -218,6 +218,9 @@ struct PrepShared {
     int hgt[MAXN], dep[MAXN];
     double rates[2 * MAXT + 1], Cs[2 * MAXT + 2], lnBt[MAXT];
     double treelen;
+    // per-time factors of the state planes
+    double eB[MAXT], eN[MAXT], eE2[MAXT], low[MAXT];
+    EmitTime et[MAXT];
 };
 
 // Lineage counts (count_lineages) with one thread per node and integer LDS
@@ -270,16 +273,19 @@ __device__ inline int node_of_state(const NodeIndex &ni, int n, int k)
     return lo;
 }
 
-// FP32 runs read every coefficient plane of a block in FP32: a copy made
-// once the block's planes are complete
-__device__ __forceinline__ void mirror_planes32(const DevInst &d, int b, int RW, int tid, int NT)
-{
-    if (!d.fp32) return;
-    __syncthreads();
-    const AW_GLOBAL double *src = gp(d.lcoef) + (size_t)b * NCOEF * RW;
-    AW_GLOBAL float *dst = gp(d.lcoef32) + (size_t)b * NCOEF * RW;
-    for (int x = tid; x < NCOEF * RW; x += NT) dst[x] = (float)src[x];
-}
+// The coefficient planes of one block, written in the run's precision:
+// FP32 runs keep only the FP32 planes (the same values as an FP64 plane
+// rounded once)
+struct PlaneOut {
+    AW_GLOBAL double *p64;
+    AW_GLOBAL float *p32;
+    int RW;
+    __device__ void put(int plane, int p, double v) const
+    {
+        if (p32) p32[(size_t)plane * RW + p] = (float)v;
+        else p64[(size_t)plane * RW + p] = v;
+    }
+};
 
 __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
                                                 const int *__restrict__ blkoff, int ninst)
@@ -452,10 +458,20 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     const Model &M = sh.M;
     const TransVec &tv = sh.tv;
     const int so = cp(d.soff)[b], S = cp(d.soff)[b + 1] - so;
-    auto coef = gp(d.lcoef) + (size_t)b * NCOEF * RW;
+    const PlaneOut coef{d.fp32 ? nullptr : gp(d.lcoef) + (size_t)b * NCOEF * RW,
+                        d.fp32 ? gp(d.lcoef32) + (size_t)b * NCOEF * RW : nullptr, RW};
     auto info = gp(d.linfo) + (size_t)b * RW;
     auto det = gp(d.ldet) + (size_t)b * RW;
 
+    // the factors of the planes that depend on the state's time only, each
+    // the expression the per-state formulas below would evaluate
+    for (int a = tid; a < T1; a += NT) {
+        sh.eB[a] = exp(tv.lnB[a]);
+        sh.eN[a] = exp(tv.lnNegG1[a]);
+        sh.eE2[a] = exp(tv.lnE2[a]);
+        sh.low[a] = a > 0 ? exp(tv.lnE2[a] + tv.lnB[a - 1]) : 0.0;
+        emit_time_terms(t, M, sh.mainlen, a, sh.et[a]);
+    }
     for (int n = tid; n < N; n += NT) {
         for (int a = sh.ag[n]; a <= sh.ni.hi[n]; a++) {
             int k = sh.ni.first[n] + a - sh.ag[n];
@@ -473,6 +489,8 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         gp(d.nmut)[2 * ((size_t)b * N + n)] = mut;
         gp(d.nmut)[2 * ((size_t)b * N + n) + 1] = nomut;
     }
+
+    __syncthreads();   // per-time factors complete
 
     // Per-state planes.  For a target state (n, b) with c = age(n), the
     // reference's same-branch entry tmatrix2[a][k] = get_time(a,b,c,same) -
@@ -492,19 +510,22 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             const int n = node_of_state(sh.ni, N, s);
             const int a = sh.ag[n] + s - sh.ni.first[n];
             const int c = sh.ag[n], h = sh.ni.hi[n];
-            const double Bc = c > 0 ? exp(tv.lnB[c - 1]) : 0.0;
-            const double low = a > 0 ? exp(tv.lnE2[a] + tv.lnB[a - 1]) : 0.0;
+            const double Bc = c > 0 ? sh.eB[c - 1] : 0.0;
+            const double low = sh.low[a];
             const double ct = c > 0 ? exp(tv.lnG4[a] + tv.lnB[c - 1]) : 0.0;
-            cf = tv.D[a] * (exp(tv.lnB[a]) - exp(tv.lnNegG1[a]) - Bc);
+            cf = tv.D[a] * (sh.eB[a] - sh.eN[a] - Bc);
             cd = tv.D[a];
-            cg = tv.E[a] * exp(tv.lnE2[a]);
+            cg = tv.E[a] * sh.eE2[a];
             ck = tv.E[a] * (low + tv.G2[a] - ct);
             // the diagonal (stay) entry dominates every state's value: it is
-            // computed as the reference computes tmatrix2, so its rounding
-            // is the reference's
-            cj = tv.get_time(a, a, c, true) - tv.get_time(a, a, 0, false);
+            // get_time(a, a, c, true) - get_time(a, a, 0, false), the
+            // reference's tmatrix2 entry, in get_time's own expressions (its
+            // low and c_term are low and ct above)
+            const double term1 = tv.D[a] * tv.E[a];
+            cj = (term1 * ((2 * (low + tv.G3[a])) - ct) + tv.norecombs[a]) -
+                 term1 * (low + tv.G3[a]);
             EmitTerms e;
-            emit_terms(t, M, sh.mainlen, n, a, e);
+            emit_terms(t, M, sh.et[a], n, a, e);
             ce = e.inv;
             for (int x = 0; x < 3; x++) {
                 gp(d.eterm)[6 * (size_t)(so + s) + x] = e.mut[x];
@@ -514,15 +535,15 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             inf = a | (a == c ? INFO_FIRST : 0) | (a == h ? INFO_LAST : 0) | INFO_VALID |
                   (c << 11) | (n << 16);
         }
-        coef[CF * RW + p] = cf;
-        coef[CD * RW + p] = cd;
-        coef[CG * RW + p] = cg;
-        coef[CK * RW + p] = ck;
-        coef[CJ * RW + p] = cj;
-        coef[CE * RW + p] = ce;
-        coef[CRR * RW + p] = 0.0;
-        coef[CCR * RW + p] = 0.0;
-        coef[CDP * RW + p] = 0.0;
+        coef.put(CF, p, cf);
+        coef.put(CD, p, cd);
+        coef.put(CG, p, cg);
+        coef.put(CK, p, ck);
+        coef.put(CJ, p, cj);
+        coef.put(CE, p, ce);
+        coef.put(CRR, p, 0.0);
+        coef.put(CCR, p, 0.0);
+        coef.put(CDP, p, 0.0);
         det[p] = -1;
         info[p] = inf;
         if (b == 0) gp(d.lprior)[p] = pr;
@@ -544,10 +565,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         gp(d.tm2s)[(size_t)b * d.TC + x] = tv.get_time(a, bb, c, true);
     }
 
-    if (b == 0) {
-        mirror_planes32(d, b, RW, tid, NT);
-        return;
-    }
+    if (b == 0) return;
     __syncthreads();   // planes cleared before the switch entries are set
 
     // switch matrix from block b-1 (calc_transition_probs_switch), entries
@@ -559,7 +577,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         int tgt = sw_determ(pt, t, sh.ni, spr, sh.map, n1, t1);
         const int p = lane_pos(j, R);
         det[p] = tgt;
-        coef[CDP * RW + p] = tgt >= 0 ? sw_determprob(pt, M, sh.PL, spr, sh.cx, n1, t1) : 0.0;
+        coef.put(CDP, p, tgt >= 0 ? sw_determprob(pt, M, sh.PL, spr, sh.cx, n1, t1) : 0.0);
     }
     if (tid == 0) {
         int rs = lookup(pt, sh.pni, spr.rn, spr.rt);
@@ -590,11 +608,11 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         int stay = lookup(t, sh.ni, sh.map[spr.rn], spr.rt);
         int esc = lookup(t, sh.ni, node2, time2);
         if (stay != -1)
-            coef[CRR * RW + lane_pos(stay, R)] = sw_recomb_recoal(
-                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, sh.cx.rpa, sh.cx.last_len);
+            coef.put(CRR, lane_pos(stay, R), sw_recomb_recoal(
+                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, sh.cx.rpa, sh.cx.last_len));
         if (esc != -1)
-            coef[CRR * RW + lane_pos(esc, R)] = sw_recomb_recoal(
-                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, spr.rt, sh.cx.last_len);
+            coef.put(CRR, lane_pos(esc, R), sw_recomb_recoal(
+                pt, M, sh.PL, spr, spr.ct, spr.rn, spr.rt, spr.rt, sh.cx.last_len));
     }
     if (sh.recoalsrc != -1) {
         const int time1 = spr.ct;
@@ -605,11 +623,10 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             if (!((n2 == mrn && t2 >= spr.rt) || (n2 == sh.node3 && t2 == time1) ||
                   (n2 == sh.rparent && t2 == time1)))
                 continue;
-            coef[CCR * RW + lane_pos(k, R)] = sw_recomb_recoal(pt, M, sh.PL, spr, t2, spr.cn,
-                                                               spr.ct, sh.cx.rpa, sh.cx.last_len);
+            coef.put(CCR, lane_pos(k, R), sw_recomb_recoal(pt, M, sh.PL, spr, t2, spr.cn, spr.ct,
+                                                           sh.cx.rpa, sh.cx.last_len));
         }
     }
-    mirror_planes32(d, b, RW, tid, NT);
 }
 
 //===========================================================================
@@ -1298,7 +1315,7 @@ __device__ __forceinline__ void tb_load_block(const DevInst &d, int b, TbLane<R>
     // re-reads its own entries, never used)
     const int bn = min(b + 1, d.B - 1);
     const AW_GLOBAL int *detn = gp(d.ldet) + (size_t)bn * RW;
-    const AW_GLOBAL double *dpn = gp(d.lcoef) + ((size_t)bn * NCOEF + CDP) * RW;
+    const AW_GLOBAL Real *dpn = planes<Real>(d, bn, RW) + CDP * RW;
     const AW_GLOBAL double *tm = gp(d.tm) + (size_t)b * TT;
     const AW_GLOBAL double *tm2 = gp(d.tm2s) + (size_t)b * d.TC;
     const int n = TT + d.TC;
@@ -1368,9 +1385,11 @@ __device__ __forceinline__ int lane_sample(const double (&pre)[R], const int (&t
 }
 
 // forward-table rows staged per traceback batch: about TB_ROWREG registers
-// of a lane hold the next batch (fewer rows per batch also mean less LDS per
-// tile, so more tiles fit on a CU)
-constexpr int TB_ROWREG = 64;
+// of a lane hold the next batch.  Fewer rows per batch mean less LDS per
+// tile: at 48 an FP32 tile with 20 time points takes about 35 KB, so four
+// tiles fit on a CU (three at 64); at 32 the batches of R = 12 and 16 are
+// too short to hide the row loads.
+constexpr int TB_ROWREG = 48;
 template <class Real, int R>
 constexpr int tb_batch()
 {
@@ -1468,7 +1487,7 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
                 // (TransMatrixSwitch::get, trans.h:201-209)
                 const int rs = cp(d.srcs)[2 * kblk], cs = cp(d.srcs)[2 * kblk + 1];
                 const int pk = lane_pos(knext, R);
-                auto cn = cp(d.lcoef) + (size_t)kblk * NCOEF * RW;
+                const AW_GLOBAL Real *cn = planes<Real>(d, kblk, RW);
                 const double rrk = cn[CRR * RW + pk], crk = cn[CCR * RW + pk];
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -2219,8 +2238,9 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             d.tm2s = p.take<double>((size_t)B * d.TC);
             d.srcs = p.take<int>(2 * (size_t)B);
             d.linfo = p.take<int>((size_t)B * RW);
-            d.lcoef = p.take<double>((size_t)B * NCOEF * RW);
+            // coefficient planes in the run's precision only
             d.fp32 = fp32 ? 1 : 0;
+            d.lcoef = fp32 ? nullptr : p.take<double>((size_t)B * NCOEF * RW);
             d.lcoef32 = fp32 ? p.take<float>((size_t)B * NCOEF * RW) : nullptr;
             d.ldet = p.take<int>((size_t)B * RW);
             d.lprior = p.take<double>(RW);

@@ -69,7 +69,7 @@ size_t Server::footprint(const HostInstance &hi, Precision prec)
     const size_t rsz = prec == Precision::FP32 ? 4 : 8;
     const size_t TC = (size_t)tc_base(T1, T1);
     size_t b = (size_t)L * RW * rsz;                          // forward table
-    b += (size_t)B * RW * (NCOEF * 8 + 4 + 4);                // planes, info, targets
+    b += (size_t)B * RW * (NCOEF * rsz + 4 + 4);              // planes, info, targets
     b += (size_t)std::max<long long>(nvar, 1) * RW * 8;        // variant rows
     b += (size_t)B * ((size_t)T1 * T1 + TC) * 8;               // transition tables
     b += (size_t)Stot * (4 + 4 + 48);                          // state tables, emission terms
