@@ -417,8 +417,8 @@ class Batch(object):
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
 
-    def tune_forward(self, candidates=((0, 0), (40000, 15000), (50000, 20000), (60000, 25000),
-                                       (80000, 25000), (100000, 40000)), tol=1e-4):
+    def tune_forward(self, candidates=((0, 0), (60000, 25000), (80000, 25000), (100000, 40000),
+                                       (120000, 50000), (140000, 60000)), tol=1e-4):
         """Autotune the FP32 forward tiles on this batch; returns the best
         (tile, warm-up) pair and {pair: forward ms}."""
         tiles = np.ascontiguousarray([c[0] for c in candidates], np.int32)
@@ -430,7 +430,7 @@ class Batch(object):
             raise RuntimeError("awamd_batch_tune_forward failed")
         return tuple(candidates[best]), {tuple(c): float(m) for c, m in zip(candidates, ms)}
 
-    def tune(self, precision=1, candidates=(0, 500, 1000, 2000, 4000, 8000)):
+    def tune(self, precision=1, candidates=(0, 1000, 2000, 4000, 8000, 12000, 16000)):
         """Autotune the traceback tile length on this batch; returns
         (best, {candidate: ms})."""
         cand = np.ascontiguousarray(candidates, np.int32)
