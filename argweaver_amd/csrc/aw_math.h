@@ -305,10 +305,14 @@ struct SwitchCtx {
     double recoals[MAXT];
 };
 
-__host__ __device__ inline void switch_ctx(const TreeView &last, const Model &m,
-                                           const Lineages &L, const Spr &spr, SwitchCtx &cx)
+// everything of the switch context but the per-time recoalescence terms
+// (sw_recoal(m, L, spr.rt, spr.ct, a, rpa) for every time a), given the
+// last tree's length
+__host__ __device__ inline void switch_ctx_sums(const TreeView &last, const Model &m,
+                                                const Lineages &L, const Spr &spr, double last_len,
+                                                SwitchCtx &cx)
 {
-    cx.last_len = tree_length(last, m.times);
+    cx.last_len = last_len;
     cx.rpa = last.age[last.parent[spr.rn]];
     const int k = spr.rt, j = spr.ct;
     double sum = 0.0;
@@ -324,7 +328,6 @@ __host__ __device__ inline void switch_ctx(const TreeView &last, const Model &m,
         sum += m.csteps[x] / (2.0 * m.pop[x / 2]);
         cx.sums2[x + 1] = sum;
     }
-    for (int a = 0; a < m.T; a++) cx.recoals[a] = sw_recoal(m, L, k, j, a, cx.rpa);
 }
 
 // Deterministic target of last-tree state (node1,time1) (get_deterministic_transitions)
@@ -399,15 +402,6 @@ __host__ __device__ inline int base_code(char c)
 struct EmitTerms {
     double mut[3], nomut[3], inv;
 };
-
-__host__ __device__ inline double emit_mainlen(const TreeView &t, const Model &m)
-{
-    const double mintime = m.times[1] * .1;
-    double len = 0.0;
-    for (int i = 0; i < t.n; i++)
-        if (i != t.root) len += fmax(m.times[t.age[t.parent[i]]] - m.times[t.age[i]], mintime);
-    return len;
-}
 
 // the terms that depend on the state's time only: the new leaf's branch
 // (dist[0]) and the invariant-column likelihood, for a state off and on the

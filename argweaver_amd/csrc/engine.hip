@@ -216,8 +216,8 @@ struct PrepShared {
     int node3, rparent;
     int scan[MAXN];
     int hgt[MAXN], dep[MAXN];
-    double rates[2 * MAXT + 1], Cs[2 * MAXT + 2], lnBt[MAXT];
-    double treelen;
+    double rates[2 * MAXT + 1], Cs[2 * MAXT + 2], lnBt[MAXT], eexp[MAXT];
+    double red[3 * 4];
     // per-time factors of the state planes
     double eB[MAXT], eN[MAXT], eE2[MAXT], low[MAXT];
     EmitTime et[MAXT];
@@ -287,6 +287,23 @@ struct PlaneOut {
     }
 };
 
+// Three sums over a 256-thread workgroup in a fixed order (a butterfly in
+// each wave, then the four wave sums in wave order), so that every run adds
+// the same terms in the same order; every thread gets the totals.
+__device__ inline void block_sums3(double (&v)[3], double *red)
+{
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+        for (int off = 32; off > 0; off >>= 1) v[x] += __shfl_xor(v[x], off);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int x = 0; x < 3; x++) red[4 * x + w] = v[x];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 3; x++) v[x] = ((red[4 * x] + red[4 * x + 1]) + red[4 * x + 2]) + red[4 * x + 3];
+}
+
 __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
                                                 const int *__restrict__ blkoff, int ninst)
 {
@@ -339,57 +356,53 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         sh.L.nb[T - 1] = 1;
         sh.PL.nb[T - 1] = 1;
     }
-    // node levels for the level-synchronous pruning of the emissions:
-    // height (leaves 0) and depth (root 0) as fixed points of parallel
-    // relaxations (monotone, so reading a neighbour's value mid-update is
-    // harmless), one round per level
-    for (int i = tid; i < N; i += NT) {
-        sh.hgt[i] = 0;
-        sh.dep[i] = 0;
-    }
+    // node levels for the level-synchronous pruning of the emissions: depth
+    // (root 0) by walking up to the root, height (leaves 0) by walking up
+    // from every leaf and raising each ancestor's height to its distance
+    // (integer LDS atomics); every walk is bounded by N
+    for (int i = tid; i < N; i += NT) sh.hgt[i] = 0;
     __syncthreads();
-    for (int it = 0; it <= N; it++) {
-        int ch = 0;
-        for (int j = tid; j < N; j += NT) {
-            if (sh.ch0[j] == -1) continue;
-            const int h = 1 + max(sh.hgt[sh.ch0[j]], sh.hgt[sh.ch1[j]]);
-            if (h != sh.hgt[j]) {
-                sh.hgt[j] = h;
-                ch = 1;
-            }
+    for (int j = tid; j < N; j += NT) {
+        int dd = 0;
+        for (int p = sh.par[j]; p != -1 && dd < N; p = sh.par[p]) dd++;
+        sh.dep[j] = dd;
+        if (sh.ch0[j] == -1) {
+            int k = 1;
+            for (int p = sh.par[j]; p != -1 && k <= N; p = sh.par[p], k++) atomicMax(&sh.hgt[p], k);
         }
-        if (!__syncthreads_or(ch)) break;
     }
-    for (int it = 0; it <= N; it++) {
-        int ch = 0;
-        for (int j = tid; j < N; j += NT) {
-            const int p = sh.par[j];
-            if (p == -1) continue;
-            const int dd = sh.dep[p] + 1;
-            if (dd != sh.dep[j]) {
-                sh.dep[j] = dd;
-                ch = 1;
+    // the block's tree lengths, fixed-order sums over the workgroup (one
+    // node per thread): get_treelen of this tree (calc_trans) and of the
+    // tree before (the switch), and the emissions' branch-length sum with
+    // its minimum branch length (emit.cpp:778-819)
+    double lens[3] = {0.0, 0.0, 0.0};
+    {
+        const double *tm_ = sh.M.times;
+        const double mintime = tm_[1] * .1;
+        if (tid < N) {
+            const int p = sh.par[tid];
+            if (p != -1) {
+                lens[0] = tm_[sh.ag[p]] - tm_[sh.ag[tid]];
+                lens[1] = fmax(tm_[sh.ag[p]] - tm_[sh.ag[tid]], mintime);
             }
+            const int pp = (b > 0) ? sh.ppar[tid] : -1;
+            if (pp != -1) lens[2] = tm_[sh.pag[pp]] - tm_[sh.pag[tid]];
         }
-        if (!__syncthreads_or(ch)) break;
+        block_sums3(lens, sh.red);   // also completes the levels
     }
     for (int i = tid; i < N; i += NT) gp(d.order)[(size_t)b * N + i] = sh.hgt[i] | (sh.dep[i] << 16);
     __syncthreads();
     // TransMatrix vectors (calc_trans) split over the workgroup with the
-    // sequential version's expressions and summation orders: the tree
-    // length in node order, the rate prefix and the lnB log-sum in time
-    // order, the other per-time terms one thread each
+    // sequential version's expressions: the rate prefix in time order on
+    // one thread, the per-time terms one thread each, the running logadd of
+    // lnB as a prefix sum of exponentials.  Meanwhile one thread computes
+    // the switch context's sums (calc_recoal_sums and friends).
+    if (tid == 0) sh.mainlen = lens[1];
+    if (tid == 128) gp(d.roots)[b] = sh.root;
+    if (tid == 192 && b > 0) switch_ctx_sums(pt, sh.M, sh.PL, spr, lens[2], sh.cx);
     {
         const Model &M = sh.M;
         double *rates = sh.rates + 1, *C = sh.Cs + 2;
-        if (tid == 0) {
-            double len = 0.0;
-            for (int i = 0; i < N; i++) {
-                const int p = t.parent[i];
-                if (p != -1) len += M.times[t.age[p]] - M.times[t.age[i]];
-            }
-            sh.treelen = len;
-        }
         if (tid == 64) rates[-1] = 0.0;
         for (int i = tid; i < 2 * T - 1; i += NT)
             rates[i] = M.csteps[i] * sh.L.nb[i / 2] / (2.0 * M.pop[i / 2]);
@@ -407,7 +420,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
         TransVec &tv = sh.tv;
         const int ri = t.age[t.root];
         const double root_age = M.times[ri];
-        const double len = sh.treelen;
+        const double len = lens[0];
         for (int bb = tid; bb < T; bb += NT) {
             tv.D[bb] = tv.E[bb] = tv.lnB[bb] = tv.lnE2[bb] = tv.lnNegG1[bb] = 0.0;
             tv.G2[bb] = tv.G3[bb] = tv.lnG4[bb] = tv.norecombs[bb] = 0.0;
@@ -434,8 +447,29 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
             tv.E[bb] = 1.0 / nc;
             tv.norecombs[bb] = exp(-fmax(M.rho * len2, M.rho));
         }
+        // the switch context's per-time recoalescence terms
+        if (b > 0)
+            for (int a = tid - 128; a >= 0 && a < T; a += NT)
+                sh.cx.recoals[a] = sw_recoal(M, sh.PL, spr.rt, spr.ct, a, sh.cx.rpa);
         __syncthreads();
-        if (tid == 0) {
+        // lnB[b] = logadd(lnB[b-1], term_b) (trans.cpp:26-115) as
+        // tmax + log(sum of exp(term_k - tmax), k <= b): the same value up to
+        // rounding; the running logadd on one thread when the terms span
+        // more than exp's range
+        double tmax = -INFINITY, tmin = INFINITY;
+        for (int k = 0; k < T - 1; k++) {
+            tmax = fmax(tmax, sh.lnBt[k]);
+            tmin = fmin(tmin, sh.lnBt[k]);
+        }
+        if (tmax - tmin < 600.0) {
+            for (int k = tid; k < T - 1; k += NT) sh.eexp[k] = exp(sh.lnBt[k] - tmax);
+            __syncthreads();
+            for (int bb = tid; bb < T - 1; bb += NT) {
+                double sum = 0.0;
+                for (int k = 0; k <= bb; k++) sum += sh.eexp[k];
+                tv.lnB[bb] = tmax + log(sum);
+            }
+        } else if (tid == 0) {
             double lb = sh.lnBt[0];
             tv.lnB[0] = lb;
             for (int bb = 1; bb < T - 1; bb++) {
@@ -443,14 +477,6 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
                 tv.lnB[bb] = lb;
             }
         }
-    }
-    // the remaining sequential pieces, one wave each
-    if (tid == 0) {
-    } else if (tid == 64) {
-        sh.mainlen = emit_mainlen(t, sh.M);
-        gp(d.roots)[b] = sh.root;
-    } else if (tid == 192 && b > 0) {
-        switch_ctx(pt, sh.M, sh.PL, spr, sh.cx);
     }
     __syncthreads();
     t.root = sh.root;
