@@ -123,8 +123,8 @@ struct BatchOptions {
     bool fixup = true;       // false: speculative traceback tiles only (diagnostics)
     // FP32 forward tiles: long instances are split into tiles (counting the
     // warm-up) of fw_tile columns at a column cost of 0.94 us, scaled by
-    // the instance's own column cost, that run concurrently; a tile after
-    // the first starts fw_warm columns early
+    // the column cost of the instance's states per lane, that run
+    // concurrently; a tile after the first starts fw_warm columns early
     // from a uniform vector, and its start is checked against the column
     // the tile before computes (relative tolerance fw_tol), a tile that
     // fails being re-run from that column.  0: one tile per instance (FP64

@@ -1238,11 +1238,9 @@ template <class Real, int NT1, bool REDO>
 __global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
                                                const FwTile *__restrict__ tiles,
                                                Real *__restrict__ bnd, double *__restrict__ bnorm,
-                                               double tol, int *__restrict__ early,
-                                               long long *__restrict__ times)
+                                               double tol, int *__restrict__ early)
 {
     extern __shared__ __align__(16) double fw_dyn[];
-    const long long t_start = wall_clock64();
     const FwTile t = tiles[blockIdx.x];
     const DevInst &d = D[t.inst];
     Real *sm = (Real *)fw_dyn;
@@ -1257,12 +1255,6 @@ __global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
     case 8: fw_instance<Real, 8, NT1>(d, sm, t, bg, ng, tol, eg); break;
     case 12: fw_instance<Real, 12, NT1>(d, sm, t, bg, ng, tol, eg); break;
     default: fw_instance<Real, 16, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    }
-    // the tile's wall-clock span (the host derives each instance's column
-    // cost from it)
-    if (times && threadIdx.x == 0) {
-        times[2 * blockIdx.x] = t_start;
-        times[2 * blockIdx.x + 1] = wall_clock64();
     }
 }
 
@@ -1668,7 +1660,6 @@ struct Ctx {
     bool init = false;
     int dev = 0;
     hipDeviceProp_t prop{};
-    int wall_khz = 0;   // rate of wall_clock64
 };
 
 Ctx &ctx()
@@ -1685,7 +1676,6 @@ int ctx_init()
     if (c.init) return 0;
     HIP_OK(hipGetDevice(&c.dev));
     HIP_OK(hipGetDeviceProperties(&c.prop, c.dev));
-    HIP_OK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, c.dev));
     c.init = true;
     return 0;
 }
@@ -1774,14 +1764,12 @@ int fw_lds_reals_r(bool fp32, int R)
 
 template <class Real>
 int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double *bnorm,
-              size_t sh, hipStream_t st, int T1max, double tol = 0.0, int *early = nullptr,
-              long long *times = nullptr)
+              size_t sh, hipStream_t st, int T1max, double tol = 0.0, int *early = nullptr)
 {
     auto kern = early ? ((T1max <= 19) ? k_fw_all<Real, 19, true> : k_fw_all<Real, MAXT - 1, true>)
                       : ((T1max <= 19) ? k_fw_all<Real, 19, false> : k_fw_all<Real, MAXT - 1, false>);
     HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, tiles, (Real *)bnd, bnorm, tol, early,
-                       times);
+    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, tiles, (Real *)bnd, bnorm, tol, early);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -1820,13 +1808,6 @@ struct Batch::Impl {
     char *d_bnd = nullptr;
     double *d_bnorm = nullptr;
     int *d_fidx = nullptr, *d_fflag = nullptr;
-    long long *d_ftime = nullptr;
-    // forward column cost of every instance (us per column) measured on the
-    // last FP32 run's main forward launch, 0 before one; the tile plan
-    // of the next FP32 run uses it
-    std::vector<double> fw_cost;
-    std::vector<long long> ftime;
-    std::vector<FwTile> ftiles;
     bool uploaded = false;
     Precision prec_done = Precision::FP64;
 
@@ -1860,7 +1841,6 @@ void Batch::clear()
     m.lnl.clear();
     m.nvar.clear();
     m.chunks.clear();
-    m.fw_cost.clear();
     m.uploaded = false;
 }
 double Batch::lnl(int i) const { return impl_->lnl[i]; }
@@ -2121,10 +2101,10 @@ int Batch::tune_forward(const int *tiles, const int *warms, int ncand, double to
     return best;
 }
 
-// FP32 forward column cost (us) by states per lane before an instance's own
-// is measured: medians of the 256-instance batch (20 time points) on MI355X.
-// Small R pays for a longer carry chain of the same-branch scan, so the
-// cost is nearly flat up to R = 6.
+// FP32 forward column cost (us) by states per lane: medians of the tiles of
+// the 256-instance batch (20 time points) on MI355X.  Small R pays for a
+// longer carry chain of the same-branch scan, so the cost is nearly flat up
+// to R = 6.
 constexpr double FW_CREF = 0.94;
 inline double fw_cost_model(int R)
 {
@@ -2152,9 +2132,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         // tiles of C0 columns at a column cost of FW_CREF us; an instance
         // whose columns cost more gets proportionally shorter tiles, so that
         // the tiles of every instance take about the same time
-        const double cost = (i < (int)m.fw_cost.size() && m.fw_cost[i] > 0) ? m.fw_cost[i]
-                                                                           : fw_cost_model(m.h[i].R);
-        const int C = C0 ? std::max(W + W / 4, (int)(C0 * FW_CREF / cost)) : 0;
+        const int C = C0 ? std::max(W + W / 4, (int)(C0 * FW_CREF / fw_cost_model(m.h[i].R))) : 0;
         if (!C || L <= C || C <= W) {
             tl.push_back(FwTile{i, 0, 0, L, 0, -1, -1});
             next.push_back(-1);
@@ -2195,22 +2173,14 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         m.d_bnorm = p.take<double>(std::max(nslot, 1));
         m.d_fidx = p.take<int>(nt);
         m.d_fflag = p.take<int>(nt);
-        m.d_ftime = p.take<long long>(2 * (size_t)nt);
         if (pass == 0 && m.fwb.ensure(p.used + 4096)) return 1;
     }
     HIP_OK(hipMemcpyAsync(m.d_fwt, tl.data(), sizeof(FwTile) * nt, hipMemcpyHostToDevice, s));
     (void)hipEventRecord(c.ev[6], s);
-    if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max, 0.0, nullptr,
-                                m.d_ftime)
+    if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max)
              : launch_fw<double>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max))
         return 1;
     (void)hipEventRecord(c.ev[7], s);
-    // the main launch's tile spans, read once the run has ended
-    m.ftiles = tl;
-    m.ftime.assign(fp32 ? 2 * (size_t)nt : 0, 0);
-    if (fp32)
-        HIP_OK(hipMemcpyAsync(m.ftime.data(), m.d_ftime, sizeof(long long) * 2 * nt,
-                              hipMemcpyDeviceToHost, s));
     // tiles whose start is to be checked: those with a boundary slot in
     std::vector<int> chk;
     for (int k = 0; k < nt; k++)
@@ -2418,22 +2388,6 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     m.prec_done = opt.prec;
-    // column cost of every instance on this run's main forward launch
-    if (!m.ftime.empty() && ctx().wall_khz > 0) {
-        std::vector<double> us(n, 0.0), cols(n, 0.0);
-        bool ok = true;
-        for (size_t k = 0; k < m.ftiles.size(); k++) {
-            const long long t0 = m.ftime[2 * k], t1 = m.ftime[2 * k + 1];
-            if (t1 <= t0) ok = false;
-            us[m.ftiles[k].inst] += (double)(t1 - t0) * 1e3 / ctx().wall_khz;
-            cols[m.ftiles[k].inst] += m.ftiles[k].c1 - m.ftiles[k].c0;
-        }
-        if (ok) {
-            m.fw_cost.assign(n, 0.0);
-            for (int i = 0; i < n; i++)
-                if (cols[i] > 0) m.fw_cost[i] = us[i] / cols[i];
-        }
-    }
     float ms;
     (void)hipEventElapsedTime(&ms, c.ev[1], c.ev[2]); rs.ms_prep = ms;
     (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]); rs.ms_emit = ms;

@@ -165,9 +165,11 @@ int awamd_batch_run(void *batch, int precision, int tile_cols, int traceback, do
  * only, for diagnostics) */
 int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int traceback, int fixup,
                          double *stats);
-/* as awamd_batch_run_opts, with FP32 forward tiles: instances longer than
- * fw_tile columns (0: off; FP64 runs ignore it) run as concurrent tiles of
- * fw_tile columns; a tile after the first starts fw_warm columns early
+/* as awamd_batch_run_opts, with FP32 forward tiles: long instances run as
+ * concurrent tiles of fw_tile columns (0: off; FP64 runs ignore it) scaled
+ * by 0.94 us over the column cost of the instance's states per lane (so
+ * that all tiles take about as long); a tile after the first starts fw_warm
+ * columns early
  * (<= 0: 25000) from a uniform vector, its first stored column is checked
  * against the column computed by the tile before at relative tolerance
  * fw_tol (<= 0: 1e-5), and a tile that fails is re-run from that column.

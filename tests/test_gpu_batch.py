@@ -106,11 +106,11 @@ def test_fp32_forward_tiles(aw, mix, warm):
     print("fp32 forward tiles warm=%d" % warm, flush=True)
     b = _batch(aw, mix)
     st = b.run(1, 300, fw_tile=900, fw_warm=warm, fw_tol=1e-5)
-    # tile lengths follow each instance's column cost, so only bounds are
-    # fixed: at least the instances longer than the longest tile are split
-    # (tiles stay within 900 / 0.94 columns), and no tile is shorter than
-    # 1.25 warm-ups (the columns of the first tile) or stores fewer than
-    # a quarter of a warm-up (the rest)
+    # tile lengths follow the column cost of each instance's states per
+    # lane, so only bounds are fixed: at least the instances longer than the
+    # longest tile are split (tiles stay within 900 / 0.94 columns), and no
+    # tile is shorter than 1.25 warm-ups (the columns of the first tile) or
+    # stores fewer than a quarter of a warm-up (the rest)
     cmax, cmin = int(900 / 0.94), max(warm + warm // 4, 1)
     lo = sum(1 + max(0, -(-(m["L"] - cmax) // (cmax - warm))) for m in mix)
     hi = sum(1 + max(0, -(-(m["L"] - cmin) // max(cmin - warm, 1))) for m in mix)
@@ -121,6 +121,15 @@ def test_fp32_forward_tiles(aw, mix, warm):
         assert abs(b.lnl(i) - it["lnl"]) <= 1e-6 * abs(it["lnl"]), (i, b.lnl(i), it["lnl"])
         assert _fp32_col_err(b, i, it) <= 2e-4, i
         assert np.mean(b.path(i) == it["path"]) >= 0.98, i
+    # the tile plan depends on the instance alone: a second run repeats the
+    # first bit for bit
+    first = [(b.lnl(i), b.path(i), b.alpha(i)) for i in range(len(mix))]
+    st2 = b.run(1, 300, fw_tile=900, fw_warm=warm, fw_tol=1e-5)
+    assert st2["fw_tiles"] == st["fw_tiles"]
+    for i in range(len(mix)):
+        assert b.lnl(i) == first[i][0], i
+        assert np.array_equal(b.path(i), first[i][1]), i
+        assert np.array_equal(b.alpha(i), first[i][2]), i
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -140,15 +149,17 @@ def test_batch_tiled_traceback_matches_untiled(aw, mix, prec):
     assert st["tb_passes"] >= 1
 
 
-@pytest.mark.parametrize("prec,tile", [(0, 0), (1, 0), (1, 400)])
-def test_alone_equals_in_mix(aw, mix, prec, tile):
-    """An instance's results do not depend on the rest of the batch."""
-    print("alone vs mix prec=%d tile=%d" % (prec, tile), flush=True)
+@pytest.mark.parametrize("prec,tile,fw", [(0, 0, 0), (1, 0, 0), (1, 400, 0), (1, 400, 900)])
+def test_alone_equals_in_mix(aw, mix, prec, tile, fw):
+    """An instance's results do not depend on the rest of the batch (FP32
+    forward tiles included: their plan depends on the instance alone)."""
+    print("alone vs mix prec=%d tile=%d fw_tile=%d" % (prec, tile, fw), flush=True)
+    kw = dict(fw_tile=fw, fw_warm=600, fw_tol=1e-5) if fw else {}
     b = _batch(aw, mix)
-    b.run(prec, tile)
+    b.run(prec, tile, **kw)
     for i in (0, 1, 3):
         one = _batch(aw, [mix[i]])
-        one.run(prec, tile)
+        one.run(prec, tile, **kw)
         assert one.lnl(0) == b.lnl(i), i
         assert np.array_equal(one.path(0), b.path(i)), i
         assert np.array_equal(one.alpha(0), b.alpha(i)), i
