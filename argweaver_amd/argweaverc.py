@@ -176,8 +176,14 @@ class IntMatrix(object):
 
     def __init__(self, arr):
         self.arr = np.ascontiguousarray(arr, dtype=np.int32)
-        rows = [self.arr[i].ctypes.data_as(c_int_p) for i in range(self.arr.shape[0])]
-        self.rows = (c_int_p * max(len(rows), 1))(*rows)
+        n = self.arr.shape[0]
+        # the row addresses, computed in one numpy expression (a ctypes
+        # pointer per row costs seconds on large ARGs)
+        addr = np.zeros(max(n, 1), np.uint64)
+        if n:
+            addr[:n] = self.arr.ctypes.data + np.arange(n, dtype=np.uint64) * self.arr.strides[0]
+        self.addr = addr
+        self.rows = (c_int_p * max(n, 1)).from_buffer(addr)
 
     @property
     def ptr(self):
@@ -233,6 +239,15 @@ def trees_internal(trees):
     seqids = np.zeros(max(nleaves, 1), np.int32)
     awamd_get_seqids(trees, _iptr(seqids))
     return maps, seqids[:nleaves]
+
+
+def seqlen_of(trees):
+    """Columns covered by the ARG's blocks (get_local_trees_blocks)."""
+    n = get_local_trees_ntrees(trees)
+    starts = np.zeros(max(n, 1), np.int32)
+    ends = np.zeros(max(n, 1), np.int32)
+    get_local_trees_blocks(trees, _iptr(starts), _iptr(ends))
+    return int(ends[n - 1] - starts[0]) if n else 0
 
 
 def num_column_states(trees, ntimes):
@@ -318,9 +333,8 @@ def thread_forward(trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=No
     times = np.ascontiguousarray(times, np.float64)
     popsizes = np.ascontiguousarray(popsizes, np.float64)
     cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
-    W, _ = num_column_states(trees, len(times))
-    L = int(sum(trees_arrays(trees)[3]))
-    alpha = np.zeros(W, np.float64) if want_alpha else None
+    L = seqlen_of(trees)
+    alpha = np.zeros(num_column_states(trees, len(times))[0], np.float64) if want_alpha else None
     path = np.zeros(L, np.int32) if uniforms is not None else None
     u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
     lnl = c_double(0.0)
