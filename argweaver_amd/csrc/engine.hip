@@ -727,14 +727,25 @@ __device__ inline void pruning_wave(const TreeView &t, const int *lvl, const cha
     }
 }
 
-__global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D,
-                                                   const int *__restrict__ voff, int ninst)
+// LDS of k_var_emit_b for trees of up to n nodes: the partials and branch
+// probabilities (FP64), the tree arrays and levels, the leaf bases.  Sized
+// by the batch's largest tree, so that small trees leave room for more
+// waves on a CU.
+inline size_t var_emit_lds(int n)
 {
-    __shared__ int par[MAXN], ch0[MAXN], ch1[MAXN], ag[MAXN], lvl[MAXN];
-    __shared__ char bases[MAXN];
-    __shared__ double muts[MAXN], nomuts[MAXN];
-    __shared__ double inner[4 * MAXN], outer[4 * MAXN];
-    __shared__ double in2[4];
+    return sizeof(double) * (10 * (size_t)n + 4) + sizeof(int) * 5 * (size_t)n + n;
+}
+
+__global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D,
+                                                   const int *__restrict__ voff, int ninst,
+                                                   int nmax)
+{
+    extern __shared__ double ve_dyn[];
+    double *inner = ve_dyn, *outer = inner + 4 * nmax;
+    double *muts = outer + 4 * nmax, *nomuts = muts + nmax, *in2 = nomuts + nmax;
+    int *par = (int *)(in2 + 4), *ch0 = par + nmax, *ch1 = ch0 + nmax, *ag = ch1 + nmax,
+        *lvl = ag + nmax;
+    char *bases = (char *)(lvl + nmax);
     const int inst = find_owner(voff, ninst, (int)blockIdx.x);
     const DevInst &d = D[inst];
     const int v = blockIdx.x - voff[inst];
@@ -2347,7 +2358,12 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
                               hipMemcpyHostToDevice, s));
     }
     (void)hipEventRecord(c.ev[2], s);
-    if (voff[n] > 0) hipLaunchKernelGGL(k_var_emit_b, dim3(voff[n]), dim3(64), 0, s, m.dD, m.d_voff, n);
+    if (voff[n] > 0) {
+        int nmax = 1;
+        for (int i = 0; i < n; i++) nmax = std::max(nmax, m.h[i].N);
+        hipLaunchKernelGGL(k_var_emit_b, dim3(voff[n]), dim3(64), var_emit_lds(nmax), s, m.dD,
+                           m.d_voff, n, nmax);
+    }
     HIP_OK(hipGetLastError());
     (void)hipEventRecord(c.ev[3], s);
 
