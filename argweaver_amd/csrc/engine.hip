@@ -214,7 +214,7 @@ struct PrepShared {
     double mainlen;
     int recombsrc, recoalsrc;
     int node3, rparent;
-    int scan[MAXN];
+    int wsum[8];
     int hgt[MAXN], dep[MAXN];
     double rates[2 * MAXT + 1], Cs[2 * MAXT + 2], lnBt[MAXT], eexp[MAXT];
     double red[3 * 4];
@@ -242,24 +242,28 @@ __device__ inline void lineages_par(const TreeView &t, int T, Lineages &L, int t
     }
 }
 
-// build_node_index with a block-wide scan (blockDim.x == MAXN)
-__device__ inline void node_index_par(const TreeView &t, int T, NodeIndex &ni, int *scan, int tid)
+// build_node_index over the workgroup, one node per thread (blockDim.x ==
+// MAXN == 256): a shuffle scan of the branch state counts in each wave,
+// then the totals of the waves before (wsum[4], one per call); the caller
+// synchronises before ni is read
+__device__ inline void node_index_par(const TreeView &t, int T, NodeIndex &ni, int *wsum, int tid)
 {
     int len = 0;
     if (tid < t.n) {
         ni.hi[tid] = t.top(tid, T);
         len = ni.hi[tid] - t.age[tid] + 1;
     }
-    scan[tid] = len;
-    __syncthreads();
-    for (int off = 1; off < MAXN; off <<= 1) {
-        const int v = tid >= off ? scan[tid - off] : 0;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
+    const int lane = tid & 63;
+    int v = len;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
     }
-    if (tid < t.n) ni.first[tid] = scan[tid] - len;
+    if (lane == 63) wsum[tid >> 6] = v;
     __syncthreads();
+    int base = 0;
+    for (int w = 0; w < (tid >> 6); w++) base += wsum[w];
+    if (tid < t.n) ni.first[tid] = base + v - len;
 }
 
 __device__ inline int node_of_state(const NodeIndex &ni, int n, int k)
@@ -350,8 +354,8 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     pt.root = (b > 0) ? sh.proot : -1;
     lineages_par(t, T, sh.L, tid, NT);
     if (b > 0) lineages_par(pt, T, sh.PL, tid, NT);
-    node_index_par(t, T, sh.ni, sh.scan, tid);
-    if (b > 0) node_index_par(pt, T, sh.pni, sh.scan, tid);
+    node_index_par(t, T, sh.ni, sh.wsum, tid);
+    if (b > 0) node_index_par(pt, T, sh.pni, sh.wsum + 4, tid);
     if (tid == 0) {
         sh.L.nb[T - 1] = 1;
         sh.PL.nb[T - 1] = 1;
