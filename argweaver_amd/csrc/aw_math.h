@@ -109,14 +109,6 @@ __host__ __device__ inline double tree_length(const TreeView &t, const double *t
     return len;
 }
 
-// number of HMM states of a tree (states.cpp:78-100)
-__host__ __device__ inline int count_states(const TreeView &t, int T)
-{
-    int s = 0;
-    for (int i = 0; i < t.n; i++) s += t.top(i, T) - t.age[i] + 1;
-    return s;
-}
-
 __host__ __device__ inline double logadd(double a, double b)
 {
     if (a == -INFINITY) return b;
@@ -221,17 +213,6 @@ struct NodeIndex {
     int first[MAXN];   // index of (node, age(node))
     int hi[MAXN];      // top time of the node's branch
 };
-
-__host__ __device__ inline int build_node_index(const TreeView &t, int T, NodeIndex &ni)
-{
-    int s = 0;
-    for (int i = 0; i < t.n; i++) {
-        ni.first[i] = s;
-        ni.hi[i] = t.top(i, T);
-        s += ni.hi[i] - t.age[i] + 1;
-    }
-    return s;
-}
 
 __host__ __device__ inline int lookup(const TreeView &t, const NodeIndex &ni, int node, int time)
 {

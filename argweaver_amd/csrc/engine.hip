@@ -242,7 +242,8 @@ __device__ inline void lineages_par(const TreeView &t, int T, Lineages &L, int t
     }
 }
 
-// build_node_index over the workgroup, one node per thread (blockDim.x ==
+// the node index (first state and top time of every node's branch,
+// states.cpp:53-75) over the workgroup, one node per thread (blockDim.x ==
 // MAXN == 256): a shuffle scan of the branch state counts in each wave,
 // then the totals of the waves before (wsum[4], one per call); the caller
 // synchronises before ni is read
