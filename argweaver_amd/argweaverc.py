@@ -444,7 +444,7 @@ class Batch(object):
             raise RuntimeError("awamd_batch_tune_forward failed")
         return tuple(candidates[best]), {tuple(c): float(m) for c, m in zip(candidates, ms)}
 
-    def tune(self, precision=1, candidates=(0, 1000, 2000, 4000, 8000, 12000, 16000)):
+    def tune(self, precision=1, candidates=(0, 2000, 4000, 8000, 10000, 12000, 14000, 16000)):
         """Autotune the traceback tile length on this batch; returns
         (best, {candidate: ms})."""
         cand = np.ascontiguousarray(candidates, np.int32)

@@ -1959,13 +1959,16 @@ int Batch::upload()
     return 0;
 }
 
-// The traceback of every instance whose forward table is resident: tiles of
-// tile_cols columns (0: one per instance), grouped by R, then fix-up passes
-// in which a tile is re-sampled from its true boundary state until it
-// coalesces with the stored path; a tile that does not coalesce passes the
-// re-run on to the tile before it.
-// Traceback tiles of every instance: tile_cols columns (0: one per
-// instance), grouped by R (cnt/off per group).  Copies the tile list to the
+// FP32 traceback column cost (us) by states per lane, measured on one
+// instance per R (20 time points) on MI355X.
+inline double tb_cost_model(int R)
+{
+    return R <= 1 ? 0.50 : R == 2 ? 0.52 : R == 4 ? 0.63 : R == 6 ? 0.75 : R == 8 ? 0.86 : R == 12 ? 1.05 : 1.25;
+}
+
+// Traceback tiles of every instance: about tile_cols columns (0: one per
+// instance; scaled by the column cost of the instance's R), grouped by R
+// (cnt/off per group).  Copies the tile list to the
 // device on the main stream; returns the LDS size of the tile kernels.
 static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t &sht)
 {
@@ -1981,7 +1984,11 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
     sht = 0;
     for (int i : byr) {
         const DevInst &d = m.h[i];
-        const int tile = tile_cols > 0 ? tile_cols : d.L;
+        // tile_cols columns at R = 12's column cost; instances with cheaper
+        // columns get proportionally longer tiles (fewer tiles, the same
+        // time per tile)
+        const int tile =
+            tile_cols > 0 ? std::max(64, (int)(tile_cols * tb_cost_model(12) / tb_cost_model(d.R))) : d.L;
         const int nt = (d.L + tile - 1) / tile;
         for (int t = 0; t < nt; t++) {
             Chunk k;
@@ -2053,7 +2060,11 @@ static int fixup_passes(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
     return 0;
 }
 
-// The traceback of every instance whose forward table is resident.
+// The traceback of every instance whose forward table is resident: the
+// speculative tiles in one launch, then fix-up passes in which a tile is
+// re-sampled from its true boundary state until it coalesces with the
+// stored path; a tile that does not coalesce passes the re-run on to the
+// tile before it.
 static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
                            RunStats &rs)
 {
