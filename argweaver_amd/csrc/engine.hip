@@ -804,18 +804,27 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
 // states, two 0/1 factors per state (not the first / not the last state of
 // its branch: they cut the prefix and suffix sums), and whether the lane
 // holds a branch start / end (the carry stops there).  Per state it also
-// keeps where its value goes in the per-time partial sums and which lane
-// holds the other-branch term of its time; padding states use time bin 31,
-// whose term is 0, so they come out 0 with no mask.
+// keeps its time, which gives both the lane holding the other-branch term
+// of its time and where its value goes in the per-time partial sums;
+// padding states use time bin 31, whose term is 0, so they come out 0 with
+// no mask.  The 0/1 factors are floats in both precisions: FP64 runs apply
+// them as selects, which keeps their registers at one per state.
 template <class Real, int R, int NT1>
 struct FwLane {
     Real cf[R], cd[R], cg[R], ck[R], cj[R], ce[R];
-    Real nf[R], nl[R];
-    int aoff[R];     // byte offset of the state's slot in the partial sums
+    float nf[R], nl[R];
     int boff[R];     // 4 * time: ds_bpermute address of the lane holding TF[time]
     Real m1, m2;
     Real tmc[NT1];   // lane b < T-1: column b of the (a,b) table (FP32)
 };
+
+// x times a 0/1 factor m (a select in FP64: the same value)
+template <class Real>
+__device__ __forceinline__ Real keep(Real x, float m)
+{
+    if constexpr (sizeof(Real) == 4) return x * m;
+    else return (m != 0.f) ? x : Real(0);
+}
 
 // The (a,b) table lives in registers (lane b holds column b) for FP32 and
 // in LDS for FP64, where registers are short.
@@ -865,9 +874,8 @@ __device__ __forceinline__ void fw_load_block(const DevInst &d, int b, FwLane<Re
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int t = ti[r] & INFO_TIME;
-        fl.nf[r] = (ti[r] & INFO_FIRST) ? Real(0) : Real(1);
-        fl.nl[r] = (ti[r] & INFO_LAST) ? Real(0) : Real(1);
-        fl.aoff[r] = (t * GCOPY + lane / (64 / GCOPY)) * (int)sizeof(double);
+        fl.nf[r] = (ti[r] & INFO_FIRST) ? 0.f : 1.f;
+        fl.nl[r] = (ti[r] & INFO_LAST) ? 0.f : 1.f;
         fl.boff[r] = t * 4;
         first |= (ti[r] & INFO_FIRST) != 0;
         last |= (ti[r] & INFO_LAST) != 0;
@@ -995,6 +1003,8 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
     // most ceil((T-2)/R) lanes from the lane holding its last state
     const int K = (T1 - 1 + R - 1) / R;
     double *GP = (double *)sm;       // 32 x GCOPY: per-time partial sums
+    // byte offset of this lane's copy within a time bin's GCOPY copies
+    const int gslot = (lane / (64 / GCOPY)) * (int)sizeof(double);
     Real *XS = (Real *)(GP + 32 * GCOPY);   // RW: previous column in state order (switch)
     Real *NW = XS + RW;              // RW + 64: switch targets, then a dummy slot per lane
     Real *VS = NW + RW + 64;         // VST rows: staged variant-column emissions
@@ -1037,7 +1047,8 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
     // column
     auto finish = [&](int col, bool first) {
 #pragma unroll
-        for (int r = 0; r < R; r++) lds_add((double *)((char *)GP + fl.aoff[r]), (double)v[r]);
+        for (int r = 0; r < R; r++)
+            lds_add((double *)((char *)GP + fl.boff[r] * (2 * GCOPY) + gslot), (double)v[r]);
         // lane l adds the copies of bin l>>1 in its half (l&1) and clears them
         // (the LDS operations of one wave run in order)
         double *gq = GP + (lane >> 1) * GCOPY + (lane & 1) * (GCOPY / 2);
@@ -1204,13 +1215,13 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
             Real p = Real(0);
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                P[r] = p * fl.nf[r];
+                P[r] = keep(p, fl.nf[r]);
                 p = fl.cf[r] * x[r] + P[r];
             }
             Real q = Real(0);
 #pragma unroll
             for (int r = R - 1; r >= 0; r--) {
-                Q[r] = q * fl.nl[r];
+                Q[r] = keep(q, fl.nl[r]);
                 q = fl.cd[r] * x[r] + Q[r];
             }
             Real c1 = Real(0), c2 = Real(0);
@@ -1223,13 +1234,13 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
             Real f1 = Real(1);
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                f1 *= fl.nf[r];
+                f1 = keep(f1, fl.nf[r]);
                 P[r] += c1 * f1;
             }
             Real f2 = Real(1);
 #pragma unroll
             for (int r = R - 1; r >= 0; r--) {
-                f2 *= fl.nl[r];
+                f2 = keep(f2, fl.nl[r]);
                 Q[r] += c2 * f2;
             }
 #pragma unroll
