@@ -30,3 +30,23 @@ def test_header_symbols_exported():
     assert len(names) >= 30, names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_tree_arrays_round_trip_and_block_extent():
+    """Host-side ARG handles (no GPU): the int** row tables the binding
+    builds carry every row, an ARG rebuilt from its arrays exports the same
+    arrays, and the columns covered by the blocks equal the summed block
+    lengths."""
+    import numpy as np
+    import argweaver_amd.argweaverc as A
+    times = np.array(A.get_time_points(20, 200e3))
+    pops = np.full(20, 1e4)
+    tr, _ = A.simulate(12, 30000, times, pops, 1.6e-8, 1.8e-8, 5)
+    P, Ag, S, bl = A.trees_arrays(tr)
+    assert P.shape[0] == len(bl) > 1
+    assert A.seqlen_of(tr) == int(bl.sum()) == 30000
+    tr2 = A.new_trees(P, Ag, S, bl)
+    for x, y in zip((P, Ag, S, bl), A.trees_arrays(tr2)):
+        assert np.array_equal(x, y)
+    A.delete_local_trees(tr2)
+    A.delete_local_trees(tr)
