@@ -101,6 +101,16 @@ awamd_num_column_states = _export(
     "awamd_num_column_states", c_longlong, [c_void_p, c_int, c_int_p])
 awamd_get_mappings = _export("awamd_get_mappings", c_int, [c_void_p, c_int_p])
 awamd_get_seqids = _export("awamd_get_seqids", c_int, [c_void_p, c_int_p])
+# ARG probability (argweaverc.py:218-241 of the reference binding)
+argweaver_likelihood = _export(
+    "arghmm_likelihood", c_double, [c_void_p, c_double_p, c_int, c_double, c_char_p_p, c_int, c_int])
+argweaver_prior_prob = _export(
+    "arghmm_prior_prob", c_double, [c_void_p, c_double_p, c_int, c_double_p, c_double])
+argweaver_tree_prior_prob = _export(
+    "arghmm_tree_prior_prob", c_double, [c_void_p, c_double_p, c_int, c_double_p])
+argweaver_joint_prob = _export(
+    "arghmm_joint_prob", c_double,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p, c_int, c_int])
 awamd_sample_recombinations = _export(
     "awamd_sample_recombinations", c_int,
     [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_int_p,
@@ -386,6 +396,36 @@ def add_thread(trees, ntimes, path, seqid, recomb_pos, recomb_node, recomb_time)
     rt = np.ascontiguousarray(recomb_time, np.int32)
     return awamd_add_thread(trees, ntimes, _iptr(path), seqid, len(rp), _iptr(rp), _iptr(rn),
                             _iptr(rt))
+
+
+def arg_likelihood(trees, seqs, times, mu):
+    """log P(sequences | ARG) (arghmm_likelihood)."""
+    times = np.ascontiguousarray(times, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_likelihood(trees, _dptr(times), len(times), mu, cs.ptr, cs.nseqs, cs.seqlen)
+
+
+def arg_prior(trees, times, popsizes, rho):
+    """log P(ARG) (arghmm_prior_prob)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return argweaver_prior_prob(trees, _dptr(times), len(times), _dptr(popsizes), rho)
+
+
+def tree_prior(trees, times, popsizes):
+    """log prior of the first local tree (arghmm_tree_prior_prob)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return argweaver_tree_prior_prob(trees, _dptr(times), len(times), _dptr(popsizes))
+
+
+def arg_joint_prob(trees, seqs, times, popsizes, mu, rho):
+    """log P(sequences, ARG) (arghmm_joint_prob)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_joint_prob(trees, _dptr(times), len(times), _dptr(popsizes), mu, rho,
+                                cs.ptr, cs.nseqs, cs.seqlen)
 
 
 def get_time_points(ntimes=20, maxtime=200e3, delta=.01):

@@ -99,6 +99,23 @@ void delete_path(int *path);
 /* src/argweaver/logging.cpp:126-127 */
 void setLogLevel(int level);
 
+/* ARG probability (host): the log-likelihood of the sequences given the
+ * local trees, the log prior of the trees and SPRs, their sum, and the log
+ * prior of the first tree.  src/argweaver/total_prob.cpp:317-326 */
+double arghmm_likelihood(LocalTrees *trees, double *times, int ntimes, double mu, char **seqs,
+                         int nseqs, int seqlen);
+/* src/argweaver/total_prob.cpp:329-342: aborts, as the reference's does */
+double arghmm_likelihood_parsimony(LocalTrees *trees, double *times, int ntimes, double mu,
+                                   char **seqs, int nseqs, int seqlen);
+/* src/argweaver/total_prob.cpp:345-352 */
+double arghmm_prior_prob(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                         double rho);
+/* src/argweaver/total_prob.cpp:355-366 */
+double arghmm_tree_prior_prob(LocalTrees *trees, double *times, int ntimes, double *popsizes);
+/* src/argweaver/total_prob.cpp:369-378 */
+double arghmm_joint_prob(LocalTrees *trees, double *times, int ntimes, double *popsizes,
+                         double mu, double rho, char **seqs, int nseqs, int seqlen);
+
 /*---------------------------------------------------------------------------
  * extensions (no counterpart in the reference)
  */
