@@ -220,13 +220,18 @@ def mcmc_rethread(A, seed, log, k=100, L=500000, nrethreads=2):
         W, _ = A.num_column_states(tr, 20)
         A.sample_thread(tr, seqs, times, pops, rho, mu)
         dt = time.perf_counter() - t0
-        out.append((dt, W, int(A.get_local_trees_ntrees(tr))))
-        log("mcmc re-thread %d: W=%.3g, %.2f s, %d local trees" % (j, W, dt, out[-1][2]))
+        # the chain's trace statistics (arghmm_likelihood / arghmm_prior_prob,
+        # outside the timed step)
+        lk, pr = A.arg_likelihood(tr, seqs, times, mu), A.arg_prior(tr, times, pops, rho)
+        out.append((dt, W, int(A.get_local_trees_ntrees(tr)), lk, pr))
+        log("mcmc re-thread %d: W=%.3g, %.2f s, %d local trees, log P(seqs | ARG) %.8g, "
+            "log P(ARG) %.8g" % (j, W, dt, out[-1][2], lk, pr))
     # the first call includes one-time allocations
-    dt, W, _ = out[-1]
+    dt, W = out[-1][0], out[-1][1]
     return dict(k=k, columns=L, popsizes="piecewise constant, 20 intervals, log-uniform 5e3..5e4",
                 rethread_s=dt, column_states=W, value=W / dt, unit="column-states/s",
-                local_trees_trace=[o[2] for o in out])
+                local_trees_trace=[o[2] for o in out], log_likelihood_trace=[o[3] for o in out],
+                log_prior_trace=[o[4] for o in out])
 
 
 def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
@@ -249,9 +254,13 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
         A.add_thread(tr, len(times), r["path"], leaf, pos, node, tm)
         dt = time.perf_counter() - t0
         steps.append(dict(leaf=leaf, seconds=dt, lnl=r["lnl"], recombinations=int(len(pos)),
-                          local_trees=int(A.get_local_trees_ntrees(tr))))
-        log("mcmc fp32 re-thread leaf %d: %.2f s, logL %.6g, %d recombinations, %d local trees"
-            % (leaf, dt, r["lnl"], len(pos), steps[-1]["local_trees"]))
+                          local_trees=int(A.get_local_trees_ntrees(tr)),
+                          log_likelihood=A.arg_likelihood(tr, seqs, times, mu),
+                          log_prior=A.arg_prior(tr, times, pops, rho)))
+        log("mcmc fp32 re-thread leaf %d: %.2f s, thread logL %.6g, %d recombinations, %d local "
+            "trees, log P(seqs | ARG) %.8g, log P(ARG) %.8g"
+            % (leaf, dt, r["lnl"], len(pos), steps[-1]["local_trees"],
+               steps[-1]["log_likelihood"], steps[-1]["log_prior"]))
     return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])))
 
 
