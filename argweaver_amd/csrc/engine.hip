@@ -1978,9 +1978,9 @@ inline double tb_cost_model(int R)
 }
 
 // Traceback tiles of every instance: about tile_cols columns (0: one per
-// instance; scaled by the column cost of the instance's R), grouped by R
-// (cnt/off per group).  Copies the tile list to the
-// device on the main stream; returns the LDS size of the tile kernels.
+// instance; scaled by the column cost of the instance's R), the instances
+// with the most states per lane first.  Copies the tile list to the device
+// on the main stream; returns the LDS size of the tile kernel (sht).
 static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t &sht)
 {
     const int n = (int)m.h.size();
