@@ -188,3 +188,24 @@ def test_forward_tile_autotuner(aw, mix):
     best, ms = b.tune_forward(candidates=cands)
     assert best in cands
     assert set(ms) == set(cands) and all(v > 0 for v in ms.values())
+
+
+def test_fp32_forward_tiles_28_time_points(aw):
+    """Forward tiles with re-runs and tiled traceback on a 28-point time grid
+    (the kernels built for up to 31 time bins): FP32 bounds against the
+    oracle's FP64 results."""
+    print("fp32 forward tiles, 28 time points", flush=True)
+    items = []
+    for k, L, seed in [(6, 1500, 31), (10, 2000, 32), (16, 1200, 33)]:
+        tr, seqs, inst, model = instance(aw, k, L, seed, ntimes=28)
+        u = np.random.RandomState(seed).uniform(size=L)
+        fw, lnl = inst.forward()
+        items.append(dict(k=k, L=L, trees=tr, seqs=seqs, inst=inst, model=model, u=u, fw=fw,
+                          lnl=lnl, path=inst.traceback(fw, u)))
+    b = _batch(aw, items)
+    st = b.run(1, 300, fw_tile=700, fw_warm=2, fw_tol=1e-5)
+    assert st["fw_tiles"] > len(items) and st["fw_reruns"] >= 1
+    for i, it in enumerate(items):
+        assert abs(b.lnl(i) - it["lnl"]) <= 1e-6 * abs(it["lnl"]), (i, b.lnl(i), it["lnl"])
+        assert _fp32_col_err(b, i, it) <= 2e-4, i
+        assert np.mean(b.path(i) == it["path"]) >= 0.98, i
