@@ -471,8 +471,9 @@ class Batch(object):
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
 
-    def tune_forward(self, candidates=((0, 0), (60000, 25000), (80000, 25000), (100000, 40000),
-                                       (120000, 50000), (140000, 60000)), tol=1e-4):
+    def tune_forward(self, candidates=((0, 0), (60000, 25000), (80000, 25000), (90000, 35000),
+                                       (100000, 40000), (110000, 45000), (120000, 50000),
+                                       (140000, 60000)), tol=1e-4):
         """Autotune the FP32 forward tiles on this batch; returns the best
         (tile, warm-up) pair and {pair: forward ms}."""
         tiles = np.ascontiguousarray([c[0] for c in candidates], np.int32)
