@@ -334,6 +334,37 @@ def sample_arg_seq(seqs, times, popsizes, rho, mu):
                                     cs.ptr, cs.nseqs, cs.seqlen)
 
 
+def sample_posterior(ptrees, ages, sprs, blocklens, seqs, times, popsizes, rho, mu):
+    """arghmm_sample_posterior: forward + traceback of sequence nleaves
+    against the ARG given as arrays; returns the path as [L, 2] (node, time)
+    rows (the reference's block conversion is kept, so only a one-tree ARG
+    gives a fully defined path)."""
+    ptrees = np.asarray(ptrees, np.int32)
+    ntrees, nnodes = ptrees.shape
+    P, A, S = IntMatrix(ptrees), IntMatrix(ages), IntMatrix(sprs)
+    bl = np.ascontiguousarray(blocklens, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    out = np.zeros((cs.seqlen, 2), np.int32)
+    res = argweaver_sample_posterior(
+        P.ptr, A.ptr, S.ptr, _iptr(bl), ntrees, nnodes, _dptr(times), len(times),
+        _dptr(popsizes), rho, mu, cs.ptr, cs.nseqs, cs.seqlen,
+        out.ctypes.data_as(POINTER(intstate)))
+    if not res:
+        raise RuntimeError("arghmm_sample_posterior failed")
+    return out
+
+
+def sample_arg_refine(seqs, times, popsizes, rho, mu, niters=1, nremove=1):
+    """arghmm_sample_arg_refine: sequential sampling, then niters Gibbs sweeps."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return argweaver_sample_arg_refine(_dptr(times), len(times), _dptr(popsizes), rho, mu,
+                                       cs.ptr, cs.nseqs, cs.seqlen, niters, nremove)
+
+
 def thread_forward(trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=None,
                    precision=0, want_alpha=True):
     """Forward (+ traceback with explicit uniforms) on the device.
@@ -450,10 +481,13 @@ class Batch(object):
         popsizes = np.ascontiguousarray(popsizes, np.float64)
         cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
         u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+        i = awamd_batch_add(self.ptr, trees, _dptr(times), len(times), _dptr(popsizes), rho,
+                            mu, cs.ptr, new_chrom, _dptr(u) if u is not None else None)
+        if i < 0:
+            raise RuntimeError("awamd_batch_add refused the instance (batch already uploaded)")
         self._keep.append((times, popsizes, cs, u))
         self.L.append(cs.seqlen)
-        return awamd_batch_add(self.ptr, trees, _dptr(times), len(times), _dptr(popsizes), rho,
-                               mu, cs.ptr, new_chrom, _dptr(u) if u is not None else None)
+        return i
 
     def upload(self):
         if awamd_batch_upload(self.ptr) != 0:
@@ -540,6 +574,8 @@ class Server(object):
         u = np.ascontiguousarray(uniforms, np.float64)
         i = awamd_server_submit(self.ptr, trees, _dptr(times), len(times), _dptr(popsizes), rho, mu,
                                 cs.ptr, new_chrom, _dptr(u))
+        if i < 0:
+            raise RuntimeError("awamd_server_submit failed (unsupported time grid)")
         self.L[i] = cs.seqlen
         return i
 
@@ -555,6 +591,8 @@ class Server(object):
         if rc == -2:
             raise TimeoutError("request %d not done after %.1f s" % (i, timeout))
         del self.L[i]
+        if rc == 3:
+            raise ValueError("request %d refused: over the kernels' limits" % i)
         if rc != 0:
             raise RuntimeError("request %d failed (%d)" % (i, rc))
         return dict(path=path, lnl=lnl.value, t_submit=t[0], t_start=t[1], t_done=t[2])

@@ -37,10 +37,15 @@ static int sample_weights(const double *w, int n)
     return n - 1;
 }
 
-static void build_instance(const ArgTrees &tr, const double *times, int ntimes,
-                           const double *popsizes, double rho, double mu, char **seqs,
-                           int new_chrom, HostInstance &hi)
+// nonzero (with a message) when the time grid does not fit the model tables
+static int build_instance(const ArgTrees &tr, const double *times, int ntimes,
+                          const double *popsizes, double rho, double mu, char **seqs,
+                          int new_chrom, HostInstance &hi)
 {
+    if (ntimes < 2 || ntimes > MAXT) {
+        fprintf(stderr, "argweaver_amd: %d time points (supported: 2..%d)\n", ntimes, MAXT);
+        return 1;
+    }
     flatten(tr, hi.ft);
     const int nleaves = tr.num_leaves();
     hi.rows.resize(nleaves + 1);
@@ -49,6 +54,7 @@ static void build_instance(const ArgTrees &tr, const double *times, int ntimes,
     std::vector<double> pop(ntimes);
     for (int i = 0; i < ntimes; i++) pop[i] = popsizes ? popsizes[i] : 0.0;
     model_init(hi.model, ntimes, times, pop.data(), rho, mu);
+    return 0;
 }
 
 // sample_recombinations (recomb.cpp:151-235) for external threading
@@ -147,7 +153,7 @@ static int sample_arg_thread(ArgTrees &tr, const double *times, int ntimes, cons
                              double rho, double mu, char **seqs, int new_chrom)
 {
     HostInstance hi;
-    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi)) return 1;
     const int L = tr.length();
     std::vector<double> u(L);
     for (int i = 0; i < L; i++) u[i] = frand();   // one draw per column, last column first
@@ -164,12 +170,13 @@ static int sample_arg_thread(ArgTrees &tr, const double *times, int ntimes, cons
     return 0;
 }
 
-static void require_gpu(const char *fn)
+// false (with a message) when no HIP device is visible: the HMM entry
+// points then return NULL instead of computing anything on the host
+static bool require_gpu(const char *fn)
 {
-    if (!device_available()) {
-        fprintf(stderr, "argweaver_amd: %s needs a HIP device and none is visible\n", fn);
-        abort();
-    }
+    if (device_available()) return true;
+    fprintf(stderr, "argweaver_amd: %s needs a HIP device and none is visible\n", fn);
+    return false;
 }
 
 }  // namespace aw
@@ -319,7 +326,7 @@ double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double
         fprintf(stderr, "argweaver_amd: internal-branch forward is not supported\n");
         return nullptr;
     }
-    require_gpu("arghmm_forward_alg");
+    if (!require_gpu("arghmm_forward_alg")) return nullptr;
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
     HostInstance hi;
     const int new_chrom = tr.num_leaves();   // ArgHmmMatrixIter default (matrices.h:224-226)
@@ -327,7 +334,7 @@ double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double
         fprintf(stderr, "argweaver_amd: arghmm_forward_alg needs %d sequences\n", new_chrom + 1);
         return nullptr;
     }
-    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi)) return nullptr;
     std::vector<int> Sv;
     long long W = instance_states(hi.ft, ntimes, &Sv);
     double *table = new double[W > 0 ? W : 1];
@@ -378,10 +385,13 @@ intstate *arghmm_sample_posterior(int **ptrees, int **ages, int **sprs, int *blo
                                   intstate *path)
 {
     (void)nseqs;
-    require_gpu("arghmm_sample_posterior");
+    if (!require_gpu("arghmm_sample_posterior")) return nullptr;
     ArgTrees *tr = trees_from_arrays(ptrees, ages, sprs, blocklens, ntrees, nnodes, 0);
     HostInstance hi;
-    build_instance(*tr, times, ntimes, popsizes, rho, mu, seqs, tr->num_leaves(), hi);
+    if (build_instance(*tr, times, ntimes, popsizes, rho, mu, seqs, tr->num_leaves(), hi)) {
+        delete tr;
+        return nullptr;
+    }
     std::vector<double> u(seqlen);
     for (int i = 0; i < seqlen; i++) u[i] = frand();
     std::vector<int> ipath(seqlen);
@@ -409,14 +419,15 @@ LocalTrees *arghmm_sample_thread(LocalTrees *trees, double *times, int ntimes, d
                                  double rho, double mu, char **seqs, int nseqs, int seqlen)
 {
     (void)seqlen;
-    require_gpu("arghmm_sample_thread");
+    if (!require_gpu("arghmm_sample_thread")) return nullptr;
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
-    if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, nseqs - 1)) abort();
+    if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, nseqs - 1)) return nullptr;
     return trees;
 }
 
-// sample_arg_seq (sample_arg.cpp:26-67), in input order
-static void sample_arg_seq(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
+// sample_arg_seq (sample_arg.cpp:26-67), in input order; nonzero when a
+// threading failed (the ARG then holds the leaves threaded so far)
+static int sample_arg_seq(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
                            double mu, char **seqs, int nseqs, int seqlen)
 {
     if (tr.num_leaves() == 0 || tr.blocks.empty()) {
@@ -430,22 +441,23 @@ static void sample_arg_seq(ArgTrees &tr, double *times, int ntimes, double *pops
     std::vector<char> has(nseqs, 0);
     for (int i = 0; i < tr.num_leaves(); i++) has[tr.seqids[i]] = 1;
     for (int i = 0; i < nseqs; i++)
-        if (!has[i] && sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, i)) abort();
+        if (!has[i] && sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, i)) return 1;
+    return 0;
 }
 
-// resample_arg (sample_arg.cpp:71-107)
-static void resample_arg(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
+// resample_arg (sample_arg.cpp:71-107); nonzero when a threading failed
+static int resample_arg(ArgTrees &tr, double *times, int ntimes, double *popsizes, double rho,
                          double mu, char **seqs, int nremove)
 {
     const int nleaves = tr.num_leaves();
     if (nremove == 1) {
         for (int chrom = 0; chrom < nleaves; chrom++) {
             remove_arg_thread(tr, chrom);
-            if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, chrom)) abort();
+            if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, chrom)) return 1;
         }
-        return;
+        return 0;
     }
-    if (nremove <= 0) return;
+    if (nremove <= 0) return 0;
     if (nremove > nleaves - 1) nremove = nleaves - 1;
     std::vector<int> avail(nleaves);
     for (int i = 0; i < nleaves; i++) avail[i] = i;
@@ -457,15 +469,19 @@ static void resample_arg(ArgTrees &tr, double *times, int ntimes, double *popsiz
     }
     for (int i = 0; i < nremove; i++) remove_arg_thread(tr, avail[i]);
     for (int i = 0; i < nremove; i++)
-        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, avail[i])) abort();
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, avail[i])) return 1;
+    return 0;
 }
 
 LocalTrees *arghmm_sample_arg_seq(double *times, int ntimes, double *popsizes, double rho, double mu,
                                   char **seqs, int nseqs, int seqlen)
 {
-    require_gpu("arghmm_sample_arg_seq");
+    if (!require_gpu("arghmm_sample_arg_seq")) return nullptr;
     ArgTrees *tr = new ArgTrees();
-    sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen);
+    if (sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen)) {
+        delete tr;
+        return nullptr;
+    }
     return reinterpret_cast<LocalTrees *>(tr);
 }
 
@@ -473,10 +489,15 @@ LocalTrees *arghmm_sample_arg_refine(double *times, int ntimes, double *popsizes
                                      double mu, char **seqs, int nseqs, int seqlen, int niters,
                                      int nremove)
 {
-    require_gpu("arghmm_sample_arg_refine");
+    if (!require_gpu("arghmm_sample_arg_refine")) return nullptr;
     ArgTrees *tr = new ArgTrees();
-    sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen);
-    for (int i = 0; i < niters; i++) resample_arg(*tr, times, ntimes, popsizes, rho, mu, seqs, nremove);
+    int rc = sample_arg_seq(*tr, times, ntimes, popsizes, rho, mu, seqs, nseqs, seqlen);
+    for (int i = 0; i < niters && !rc; i++)
+        rc = resample_arg(*tr, times, ntimes, popsizes, rho, mu, seqs, nremove);
+    if (rc) {
+        delete tr;
+        return nullptr;
+    }
     return reinterpret_cast<LocalTrees *>(tr);
 }
 
@@ -485,12 +506,13 @@ LocalTrees *arghmm_resample_arg(LocalTrees *trees, double *times, int ntimes, do
                                 int niters, int nremove)
 {
     (void)seqlen;
-    require_gpu("arghmm_resample_arg");
+    if (!require_gpu("arghmm_resample_arg")) return nullptr;
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
     // arghmm_complete_arg (sample_arg.cpp:655-662)
     for (int c = tr.num_leaves(); c < nseqs; c++)
-        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) abort();
-    for (int i = 0; i < niters; i++) resample_arg(tr, times, ntimes, popsizes, rho, mu, seqs, nremove);
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) return nullptr;
+    for (int i = 0; i < niters; i++)
+        if (resample_arg(tr, times, ntimes, popsizes, rho, mu, seqs, nremove)) return nullptr;
     return trees;
 }
 
@@ -518,7 +540,7 @@ int awamd_thread_forward(LocalTrees *trees, const double *times, int ntimes,
     }
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
     HostInstance hi;
-    build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi);
+    if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi)) return -1;
     RunStats st;
     int rc = run_instance(hi, uniforms, path_out, alpha_out, lnl_out, &st,
                           precision == 1 ? Precision::FP32 : Precision::FP64);
@@ -582,8 +604,9 @@ int awamd_batch_add(void *b, LocalTrees *trees, const double *times, int ntimes,
 {
     AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
     HostInstance hi;
-    build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu, seqs,
-                   new_chrom, hi);
+    if (build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu,
+                       seqs, new_chrom, hi))
+        return -1;
     return ab.batch.add(hi, uniforms);
 }
 
@@ -757,8 +780,9 @@ int awamd_server_submit(void *server, LocalTrees *trees, const double *times, in
                         int new_chrom, const double *uniforms)
 {
     HostInstance hi;
-    build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu, seqs,
-                   new_chrom, hi);
+    if (build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu,
+                       seqs, new_chrom, hi))
+        return -1;
     return reinterpret_cast<Server *>(server)->submit(hi, uniforms);
 }
 

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "aw_math.h"
@@ -178,6 +179,10 @@ private:
 int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha,
                  double *lnl, RunStats *stats, Precision prec = Precision::FP64,
                  const double *prior = nullptr, int tile_cols = 0);
+
+// 0 when the instance fits the kernels' limits (nodes per tree, time points,
+// states per block); else 1, with the reason in msg.
+int instance_supported(const HostInstance &hi, std::string *msg = nullptr);
 
 // Number of column-states W and per-block state counts (host, O(B*N)).
 long long instance_states(const FlatTrees &ft, int T, std::vector<int> *S);

@@ -1707,9 +1707,21 @@ int ctx_init()
     return 0;
 }
 
+// Device work of all batches is serialised by one process-wide lock: the
+// upload, run, tuning and copy-out methods of Batch take it, so batches
+// driven from different host threads run one after another, and the
+// per-launch hipFuncSetAttribute calls on the shared kernel symbols never
+// race.  Recursive because run() uploads a batch that is not yet uploaded.
+std::recursive_mutex &device_mutex()
+{
+    static std::recursive_mutex mu;
+    return mu;
+}
+#define DEVICE_LOCK std::lock_guard<std::recursive_mutex> device_lock_(device_mutex())
+
 // The stream and events of one batch.  Each batch owns its own stream (at
-// the default priority), so batches driven from different host threads
-// overlap on the device.
+// the default priority); batches never overlap on the device (see
+// device_mutex).
 struct Streams {
     bool init = false;
     hipStream_t main = nullptr;
@@ -1876,6 +1888,17 @@ long long Batch::W(int i) const { return impl_->W[i]; }
 int Batch::add(const HostInstance &hi, const double *u, const double *prior)
 {
     Impl &m = *impl_;
+    // upload() releases the host rows of the instances it copied, so a batch
+    // that was uploaded takes no more instances until clear()
+    if (m.uploaded) {
+        fprintf(stderr, "argweaver_amd: Batch::add after upload (clear() the batch first)\n");
+        return -1;
+    }
+    std::string why;
+    if (instance_supported(hi, &why)) {
+        fprintf(stderr, "argweaver_amd: Batch::add refused the instance: %s\n", why.c_str());
+        return -1;
+    }
     m.hosts.push_back(hi);
     m.us.emplace_back();
     if (u) m.us.back().assign(u, u + hi.ft.seqlen);
@@ -1892,6 +1915,7 @@ int Batch::add(const HostInstance &hi, const double *u, const double *prior)
 // Inputs: trees, sequences, uniforms and the model of every instance.
 int Batch::upload()
 {
+    DEVICE_LOCK;
     if (ctx_init()) return 1;
     Impl &m = *impl_;
     if (m.st.create()) return 1;
@@ -2091,6 +2115,7 @@ static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, b
 
 int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out)
 {
+    DEVICE_LOCK;
     BatchOptions opt;
     opt.prec = prec;
     opt.traceback = false;
@@ -2119,6 +2144,7 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
 
 int Batch::tune_forward(const int *tiles, const int *warms, int ncand, double tol, double *ms_out)
 {
+    DEVICE_LOCK;
     int best = -1;
     double best_ms = 0;
     for (int i = 0; i < ncand; i++) {
@@ -2274,6 +2300,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
 
 int Batch::run(const BatchOptions &opt, RunStats *st)
 {
+    DEVICE_LOCK;
     if (ctx_init()) return 1;
     Impl &m = *impl_;
     if (!m.uploaded && upload()) return 1;
@@ -2454,6 +2481,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
 
 int Batch::copy_path(int i, int *path) const
 {
+    DEVICE_LOCK;
     const Impl &m = *impl_;
     HIP_OK(hipMemcpy(path, m.h[i].path, sizeof(int) * m.h[i].L, hipMemcpyDeviceToHost));
     return 0;
@@ -2461,6 +2489,7 @@ int Batch::copy_path(int i, int *path) const
 
 int Batch::copy_norms(int i, double *norms) const
 {
+    DEVICE_LOCK;
     const Impl &m = *impl_;
     HIP_OK(hipMemcpy(norms, m.h[i].norms, sizeof(double) * m.h[i].L, hipMemcpyDeviceToHost));
     return 0;
@@ -2470,6 +2499,7 @@ int Batch::copy_norms(int i, double *norms) const
 // reference's table layout) of the lane-run table
 int Batch::copy_alpha(int i, double *alpha) const
 {
+    DEVICE_LOCK;
     const Impl &m = *impl_;
     const DevInst &d = m.h[i];
     const FlatTrees &ft = m.hosts[i].ft;
@@ -2516,6 +2546,27 @@ long long instance_states(const FlatTrees &ft, int T, std::vector<int> *Sv)
     return W;
 }
 
+int instance_supported(const HostInstance &hi, std::string *msg)
+{
+    char buf[160];
+    buf[0] = 0;
+    const FlatTrees &ft = hi.ft;
+    if (ft.nnodes > MAXN || hi.model.T > MAXT) {
+        snprintf(buf, sizeof buf, "instance has %d nodes and %d time points (limits %d and %d)",
+                 ft.nnodes, hi.model.T, MAXN, MAXT);
+    } else {
+        std::vector<int> Sv;
+        instance_states(ft, hi.model.T, &Sv);
+        int Smax = 1;
+        for (int S : Sv) Smax = std::max(Smax, S);
+        if (lanes_per_state_r(Smax) < 0)
+            snprintf(buf, sizeof buf, "a block has %d states (limit %d per block)", Smax,
+                     64 * RVALS[NRV - 1]);
+    }
+    if (msg) *msg = buf;
+    return buf[0] ? 1 : 0;
+}
+
 bool device_available()
 {
     int n = 0;
@@ -2551,7 +2602,7 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
     static thread_local Batch *btp = new Batch();
     Batch &bt = *btp;
     bt.clear();
-    bt.add(hi, u, prior);
+    if (bt.add(hi, u, prior) < 0) return 1;
     if (bt.upload()) return 1;
     BatchOptions opt;
     opt.prec = prec;

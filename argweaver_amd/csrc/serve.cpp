@@ -80,6 +80,21 @@ size_t Server::footprint(const HostInstance &hi, Precision prec)
 
 int Server::submit(const HostInstance &hi, const double *u)
 {
+    // a request the kernels cannot run fails on its own, at once, instead of
+    // failing the batch it would join
+    std::string why;
+    if (instance_supported(hi, &why)) {
+        fprintf(stderr, "argweaver_amd: request refused: %s\n", why.c_str());
+        std::lock_guard<std::mutex> lk(mu_);
+        ServeResult res;
+        const int id = next_id_++;
+        res.id = id;
+        res.status = kRefused;
+        res.t_submit = res.t_start = res.t_done = now();
+        done_[id] = std::move(res);
+        cv_done_.notify_all();
+        return id;
+    }
     Request *r = new Request;
     r->hi = hi;
     const int L = hi.ft.seqlen;

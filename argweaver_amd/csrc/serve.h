@@ -30,9 +30,13 @@ struct ServeOptions {
     BatchOptions run;                      // precision and traceback tiles
 };
 
+// status of a request the kernels cannot run (over the node, time-point or
+// states-per-block limits), refused at submission
+constexpr int kRefused = 3;
+
 struct ServeResult {
     int id = -1;
-    int status = 0;          // 0 ok, nonzero: the batch failed
+    int status = 0;          // 0 ok, kRefused, else the batch failed
     double lnl = 0.0;
     std::vector<int> path;
     long long W = 0;         // column-states of the request

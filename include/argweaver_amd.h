@@ -12,10 +12,13 @@
  * reference: explicit uniform streams, FP32 mode, batched and served
  * threading, and the synthetic workload generator.
  *
- * Errors follow the reference: invalid input aborts the process (the
- * reference asserts), unsupported modes print to stderr and return NULL.
- * Calls that run the HMM need a visible HIP device and abort without one.
+ * Errors: an invalid ARG aborts the process, as the reference's asserts
+ * do.  Calls that run the HMM print to stderr and return NULL (or a
+ * nonzero status) when no HIP device is visible, when an instance exceeds
+ * the kernels' limits (256 nodes per tree, 32 time points, 1024 states per
+ * block) or when a device call fails, where the reference would abort.
  * Not thread-safe; the traceback draws from the process-global rand().
+ * Device work of all calls is serialised inside the library.
  */
 #ifndef ARGWEAVER_AMD_H
 #define ARGWEAVER_AMD_H
@@ -162,7 +165,9 @@ LocalTrees *awamd_add_thread(LocalTrees *trees, int ntimes, const int *path, int
 
 /* Batches of independent threading instances.  awamd_batch_add stages an
  * instance (the ARG is flattened at once; sequence rows and uniforms must
- * stay valid until awamd_batch_upload, which copies every input to HBM).
+ * stay valid until awamd_batch_upload, which copies every input to HBM;
+ * returns -1, adding nothing, after an upload or for an instance over the
+ * kernels' limits).
  * awamd_batch_run executes classification, block preparation, emissions,
  * forward and traceback for all instances.  precision 0: FP64 forward table
  * (the drop-in arithmetic); precision 1: FP32 table and recursion with
@@ -223,11 +228,14 @@ LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
  * fits hbm_budget_gb (at most max_batch; a request larger than the budget
  * runs alone) and runs them as one batch, so requests that arrive while a
  * batch runs share the next launch.  submit copies the sequence rows and
- * uniforms and returns an id; wait blocks for it (at most timeout_s seconds
+ * uniforms and returns an id (-1 for a time grid of more than 32 points,
+ * which is not queued); wait blocks for it (at most timeout_s seconds
  * when timeout_s > 0) and fills path[seqlen], the log-likelihood and
  * times3 = {submitted, started, finished} in seconds on the server clock.
  * It returns 0 on success, -2 on time-out (the request stays pending, wait
- * again), -1 for an unknown id, else the failed batch's status.  A
+ * again), -1 for an unknown id, 3 for a request refused at submission
+ * (over the kernels' node, time-point or states-per-block limits; it
+ * fails alone and never joins a batch), else the failed batch's status.  A
  * request's result does not depend on the batch it ran in.  FP32 servers
  * run requests longer than 50000 columns as forward tiles (as
  * awamd_batch_run_ex with fw_tile 50000, fw_warm 20000, fw_tol 1e-4). */

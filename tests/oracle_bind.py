@@ -291,6 +291,32 @@ def ref_sample_arg_seq(seqs, times, popsizes, rho, mu):
                                        c_double(mu), s.ptr, s.arr.shape[0], s.arr.shape[1])
 
 
+def ref_sample_arg_refine(seqs, times, popsizes, rho, mu, niters, nremove):
+    s = _Seqs(seqs)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    fn = ref().arghmm_sample_arg_refine
+    fn.restype = c_void_p
+    return fn(_dp(times), len(times), _dp(popsizes), c_double(rho), c_double(mu), s.ptr,
+              s.arr.shape[0], s.arr.shape[1], niters, nremove)
+
+
+def ref_sample_posterior(ptrees, ages, sprs, blocklens, seqs, times, popsizes, rho, mu):
+    """arghmm_sample_posterior of the compiled reference, as [L, 2] rows."""
+    P, A, S = _IntMat(ptrees), _IntMat(ages), _IntMat(sprs)
+    bl = np.ascontiguousarray(blocklens, np.int32)
+    s = _Seqs(seqs)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    out = np.zeros((s.arr.shape[1], 2), np.int32)
+    fn = ref().arghmm_sample_posterior
+    fn.restype = c_void_p
+    fn(P.ptr, A.ptr, S.ptr, _ip(bl), P.arr.shape[0], P.arr.shape[1], _dp(times), len(times),
+       _dp(popsizes), c_double(rho), c_double(mu), s.ptr, s.arr.shape[0], s.arr.shape[1],
+       out.ctypes.data_as(c_int_p))
+    return out
+
+
 _libc = ctypes.CDLL(None)
 _libc.rand.restype = c_int
 

@@ -72,3 +72,38 @@ def test_budget_admission(aw):
     spans = sorted((r["t_start"], r["t_done"]) for r in res)
     for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
         assert s1 >= e0 - 1e-6
+
+
+def test_oversized_request_fails_alone(aw):
+    """A request over the kernels' limits (257 nodes per tree) is refused at
+    submission with its own status; the valid request queued beside it runs."""
+    print("serve oversized + valid", flush=True)
+    items = _items(aw)[:2]
+    big_tr, big_seqs, _, big_model = instance(aw, 130, 300, 81)
+    srv = aw.Server(hbm_budget_gb=8.0, precision=1, tile_cols=0, max_batch=8)
+    i0 = srv.submit(items[0]["trees"], items[0]["seqs"], *items[0]["model"], items[0]["k"] - 1,
+                    items[0]["u"])
+    ib = srv.submit(big_tr, big_seqs, *big_model, 129, np.random.RandomState(81).uniform(size=300))
+    i1 = srv.submit(items[1]["trees"], items[1]["seqs"], *items[1]["model"], items[1]["k"] - 1,
+                    items[1]["u"])
+    with pytest.raises(ValueError):
+        srv.wait(ib, timeout=60)
+    r0, r1 = srv.wait(i0, timeout=60), srv.wait(i1, timeout=60)
+    srv.close()
+    for it, r in zip(items, (r0, r1)):
+        path, lnl = _alone(aw, it, 1, 0)
+        assert r["lnl"] == lnl and np.array_equal(r["path"], path)
+
+
+def test_batch_add_after_upload_refused(aw):
+    """upload() releases the staged host rows, so a later add() is refused
+    (it would otherwise plan an instance with no rows)."""
+    it = _items(aw)[0]
+    b = aw.Batch()
+    b.add(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+    b.upload()
+    with pytest.raises(RuntimeError):
+        b.add(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+    b.run(1, 0)
+    path, lnl = _alone(aw, it, 1, 0)
+    assert b.lnl(0) == lnl and np.array_equal(b.path(0), path)
