@@ -97,6 +97,10 @@ awamd_thread_forward = _export(
     "awamd_thread_forward", c_int,
     [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p,
      c_int, c_double_p, c_int, c_double_p, c_int_p, c_double_p, c_double_p])
+awamd_thread_forward_noalloc = _export(
+    "awamd_thread_forward_noalloc", c_int,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double, c_char_p_p,
+     c_int, c_double_p, c_int, c_double_p, c_int_p, c_double_p, c_double_p])
 awamd_num_column_states = _export(
     "awamd_num_column_states", c_longlong, [c_void_p, c_int, c_int_p])
 awamd_get_mappings = _export("awamd_get_mappings", c_int, [c_void_p, c_int_p])
@@ -366,9 +370,11 @@ def sample_arg_refine(seqs, times, popsizes, rho, mu, niters=1, nremove=1):
 
 
 def thread_forward(trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=None,
-                   precision=0, want_alpha=True):
+                   precision=0, want_alpha=True, noalloc=False):
     """Forward (+ traceback with explicit uniforms) on the device.
 
+    noalloc: run through awamd_thread_forward_noalloc (the fixed per-GPU
+    workspace instead of allocated device buffers).
     Returns dict(alpha=[W] FP64 block-major table or None, path=[L] or None,
     lnl=float, ms=[prep, emit, forward, traceback])."""
     times = np.ascontiguousarray(times, np.float64)
@@ -380,13 +386,14 @@ def thread_forward(trees, seqs, times, popsizes, rho, mu, new_chrom, uniforms=No
     u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
     lnl = c_double(0.0)
     ms = np.zeros(4, np.float64)
-    rc = awamd_thread_forward(
+    fn = awamd_thread_forward_noalloc if noalloc else awamd_thread_forward
+    rc = fn(
         trees, _dptr(times), len(times), _dptr(popsizes), rho, mu, cs.ptr, new_chrom,
         _dptr(u) if u is not None else None, precision,
         _dptr(alpha) if alpha is not None else None,
         _iptr(path) if path is not None else None, ctypes.byref(lnl), _dptr(ms))
     if rc != 0:
-        raise RuntimeError("awamd_thread_forward failed (%d)" % rc)
+        raise RuntimeError("%s failed (%d)" % (fn.__name__, rc))
     return dict(alpha=alpha, path=path, lnl=lnl.value, ms=ms)
 
 

@@ -553,6 +553,31 @@ int awamd_thread_forward(LocalTrees *trees, const double *times, int ntimes,
     return rc;
 }
 
+int awamd_thread_forward_noalloc(LocalTrees *trees, const double *times, int ntimes,
+                                 const double *popsizes, double rho, double mu, char **seqs,
+                                 int new_chrom, const double *uniforms, int precision,
+                                 double *alpha_out, int *path_out, double *lnl_out,
+                                 double *ms_out)
+{
+    if (!device_available()) {
+        fprintf(stderr, "argweaver_amd: awamd_thread_forward_noalloc needs a HIP device\n");
+        return -1;
+    }
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    HostInstance hi;
+    if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi)) return -1;
+    RunStats st;
+    int rc = run_instance_noalloc(hi, uniforms, path_out, alpha_out, lnl_out, &st,
+                                  precision == 1 ? Precision::FP32 : Precision::FP64);
+    if (ms_out) {
+        ms_out[0] = st.ms_prep;
+        ms_out[1] = st.ms_emit;
+        ms_out[2] = st.ms_forward;
+        ms_out[3] = st.ms_traceback;
+    }
+    return rc;
+}
+
 long long awamd_num_column_states(LocalTrees *trees, int ntimes, int *nstates_per_block)
 {
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);

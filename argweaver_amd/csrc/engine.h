@@ -145,6 +145,11 @@ public:
     ~Batch();
     Batch(const Batch &) = delete;
     Batch &operator=(const Batch &) = delete;
+    // Take every device buffer from [base, base + cap) instead of hipMalloc
+    // (re-carved from the start on clear(); a run that needs more fails).
+    void use_workspace(void *base, size_t cap);
+    // create the batch's stream and events now (else the first upload does)
+    int init_streams();
     // the instance's row pointers and uniforms must stay valid until upload()
     int add(const HostInstance &hi, const double *u, const double *prior = nullptr);
     // drop the instances, keep the device buffers for the next set
@@ -183,6 +188,17 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
 // 0 when the instance fits the kernels' limits (nodes per tree, time points,
 // states per block); else 1, with the reason in msg.
 int instance_supported(const HostInstance &hi, std::string *msg = nullptr);
+
+// Bytes of the fixed per-GPU workspace of run_instance_noalloc.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+
+// run_instance (FP64 or FP32, no prior) on a fixed workspace of
+// kNoAllocBytes per GPU instead of allocated buffers: after the first call
+// on a device (which finds the workspace and creates the stream) it calls
+// no device allocator.  Fails (1, with a message) for an instance that
+// needs more.  Calls are serialised by the library's device lock.
+int run_instance_noalloc(const HostInstance &hi, const double *u, int *path, double *alpha,
+                         double *lnl, RunStats *stats, Precision prec);
 
 // Number of column-states W and per-block state counts (host, O(B*N)).
 long long instance_states(const FlatTrees &ft, int T, std::vector<int> *S);

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -1744,13 +1745,37 @@ struct Streams {
     }
 };
 
-// grow-only device buffer
+// A caller-provided device region that buffers are carved from instead of
+// hipMalloc (the no-allocation entry point); reset() before each use.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void reset() { used = 0; }
+};
+
+// grow-only device buffer; with an arena it takes its memory from the arena
+// (and never frees it)
 struct DevBuf {
     char *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;
     int ensure(size_t n)
     {
         if (n <= cap) return 0;
+        if (arena) {
+            const size_t off = (arena->used + 255) & ~size_t(255);
+            if (off + n > arena->cap) {
+                fprintf(stderr,
+                        "argweaver_amd: the instance needs more than the %zu bytes of the "
+                        "no-allocation workspace\n",
+                        arena->cap);
+                return 1;
+            }
+            p = arena->base + off;
+            cap = n;
+            arena->used = off + n;
+            return 0;
+        }
         if (p) HIP_OK(hipFree(p));
         p = nullptr;
         cap = 0;
@@ -1760,7 +1785,7 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -1837,6 +1862,7 @@ struct Batch::Impl {
     std::vector<int> nvar;
     std::vector<Chunk> chunks;
     DevBuf in, work, tab, aux, vbuf, tiles, fwb;
+    Arena arena;   // base == nullptr: buffers come from hipMalloc
     Streams st;
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
@@ -1867,9 +1893,35 @@ Batch::Batch() : impl_(new Impl) {}
 Batch::~Batch() { delete impl_; }
 int Batch::size() const { return (int)impl_->hosts.size(); }
 
+void Batch::use_workspace(void *base, size_t cap)
+{
+    Impl &m = *impl_;
+    m.arena.base = static_cast<char *>(base);
+    m.arena.cap = cap;
+    m.arena.reset();
+    for (DevBuf *b : {&m.in, &m.work, &m.tab, &m.aux, &m.vbuf, &m.tiles, &m.fwb}) {
+        b->release();
+        b->arena = &m.arena;
+    }
+}
+
+int Batch::init_streams()
+{
+    if (ctx_init()) return 1;
+    return impl_->st.create();
+}
+
 void Batch::clear()
 {
     Impl &m = *impl_;
+    if (m.arena.base) {
+        // a workspace batch carves its buffers again from the start
+        m.arena.reset();
+        for (DevBuf *b : {&m.in, &m.work, &m.tab, &m.aux, &m.vbuf, &m.tiles, &m.fwb}) {
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    }
     m.hosts.clear();
     m.us.clear();
     m.priors.clear();
@@ -2591,16 +2643,11 @@ int device_info(char *buf, int n)
     return 0;
 }
 
-// one instance = a batch of one; the batch (its device buffers and streams)
-// is kept per host thread, so repeated calls (every re-threading of
-// resample_arg) reuse the allocations
-int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha, double *lnl,
-                 RunStats *stats, Precision prec, const double *prior, int tile_cols)
+// one instance on the given batch (cleared first)
+static int run_one(Batch &bt, const HostInstance &hi, const double *u, int *path, double *alpha,
+                   double *lnl, RunStats *stats, Precision prec, const double *prior,
+                   int tile_cols)
 {
-    // never destroyed: releasing HIP objects from exit-time destructors can
-    // run after the runtime itself has shut down
-    static thread_local Batch *btp = new Batch();
-    Batch &bt = *btp;
     bt.clear();
     if (bt.add(hi, u, prior) < 0) return 1;
     if (bt.upload()) return 1;
@@ -2622,6 +2669,72 @@ int run_instance(const HostInstance &hi, const double *u, int *path, double *alp
     if (alpha && bt.copy_alpha(0, alpha)) return 1;
     if (lnl) *lnl = bt.lnl(0);
     return 0;
+}
+
+// one instance = a batch of one; the batch (its device buffers and streams)
+// is kept per host thread, so repeated calls (every re-threading of
+// resample_arg) reuse the allocations
+int run_instance(const HostInstance &hi, const double *u, int *path, double *alpha, double *lnl,
+                 RunStats *stats, Precision prec, const double *prior, int tile_cols)
+{
+    // never destroyed: releasing HIP objects from exit-time destructors can
+    // run after the runtime itself has shut down
+    static thread_local Batch *btp = new Batch();
+    return run_one(*btp, hi, u, path, alpha, lnl, stats, prec, prior, tile_cols);
+}
+
+// The no-allocation workspace: one fixed device array, of which every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never uses the name `scratch` as an address (on the host it is a stand-in
+// symbol, not device memory): the copy on each device is found once, with
+// that device current, by hipGetSymbolAddress, and that pointer is what the
+// batch's buffers are carved from.
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocDevice {
+    char *ptr = nullptr;     // device address of this GPU's scratch
+    Batch *batch = nullptr;  // never destroyed (see run_instance)
+};
+}  // namespace
+
+// Set-up of the current device, once: the scratch address and a batch whose
+// buffers live in it, with its stream and events created here, so that the
+// calls after it allocate nothing.
+static int noalloc_device(NoAllocDevice *&out)
+{
+    static std::mutex mu;
+    static std::map<int, NoAllocDevice> devs;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = devs.find(dev);
+    if (it == devs.end()) {
+        NoAllocDevice d;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "argweaver_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+                    dev, hipGetErrorString(e));
+            return 1;
+        }
+        d.ptr = static_cast<char *>(p);
+        d.batch = new Batch();
+        d.batch->use_workspace(d.ptr, kNoAllocBytes);
+        if (d.batch->init_streams()) return 1;
+        it = devs.emplace(dev, d).first;
+    }
+    out = &it->second;
+    return 0;
+}
+
+int run_instance_noalloc(const HostInstance &hi, const double *u, int *path, double *alpha,
+                         double *lnl, RunStats *stats, Precision prec)
+{
+    NoAllocDevice *d = nullptr;
+    if (noalloc_device(d)) return 1;
+    DEVICE_LOCK;   // calls on one GPU share its scratch: one at a time
+    return run_one(*d->batch, hi, u, path, alpha, lnl, stats, prec, nullptr, 0);
 }
 
 }  // namespace aw

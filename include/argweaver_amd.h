@@ -139,6 +139,25 @@ int awamd_thread_forward(LocalTrees *trees, const double *times, int ntimes,
                          int new_chrom, const double *uniforms, int precision,
                          double *alpha_out, int *path_out, double *lnl_out, double *ms_out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  awamd_thread_forward with every device buffer taken
+ * from a fixed 64 MiB workspace that each GPU holds for the life of the
+ * process, instead of allocated: after the first call on a device (which
+ * locates the workspace and creates the library's stream) no call
+ * allocates device memory.  The same kernels and schedule as
+ * awamd_thread_forward, so the same results.  Returns 1, with a message on
+ * stderr, for an instance whose buffers need more than the workspace
+ * (roughly seqlen * 64R * 8 bytes of forward table plus the per-block
+ * planes; config-1-sized instances fit).  Every call on a GPU shares its
+ * workspace: the library serialises calls in one process, and results are
+ * copied out before a call returns. */
+int awamd_thread_forward_noalloc(LocalTrees *trees, const double *times, int ntimes,
+                                 const double *popsizes, double rho, double mu, char **seqs,
+                                 int new_chrom, const double *uniforms, int precision,
+                                 double *alpha_out, int *path_out, double *lnl_out,
+                                 double *ms_out);
+
 /* Column-states W = sum_b blocklen_b * S_b and, optionally, S_b per block. */
 long long awamd_num_column_states(LocalTrees *trees, int ntimes, int *nstates_per_block);
 /* Node mapping of every block (nnodes ints per block, -1 rows for block 0). */
