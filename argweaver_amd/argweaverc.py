@@ -91,6 +91,54 @@ argweaver_resample_arg = _export(
     [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
      c_char_p_p, c_int, c_int, c_int, c_int])
 
+# dense matrices, checks and the generic HMM (argweaverc.py:40-128)
+new_transition_probs = _export(
+    "new_transition_probs", c_double_p_p,
+    [c_int, c_int_p, c_int_p, c_double, POINTER(intstate), c_int, c_int, c_double_p, c_double_p,
+     c_int_p, c_int_p, c_int_p, c_double_p, c_double])
+new_transition_probs_switch = _export(
+    "new_transition_probs_switch", c_double_p_p,
+    [c_int_p, c_int_p, c_int, c_int, c_int, c_int, c_int, c_int_p, c_int_p, c_double, c_double,
+     POINTER(intstate), c_int, POINTER(intstate), c_int, c_int, c_double_p, c_double_p, c_int_p,
+     c_int_p, c_int_p, c_double_p, c_double])
+delete_transition_probs = _export("delete_transition_probs", None, [c_double_p_p, c_int])
+argweaver_assert_transmat = _export(
+    "arghmm_assert_transmat", c_bool,
+    [c_int, c_int_p, c_int_p, c_int, c_double_p, c_double_p, c_double])
+argweaver_assert_transmat_switch = _export(
+    "arghmm_assert_transmat_switch", c_bool,
+    [c_int, c_int_p, c_int_p, c_int, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_double])
+new_emissions = _export(
+    "new_emissions", c_double_p_p,
+    [POINTER(intstate), c_int, c_int_p, c_int, c_int_p, c_char_p_p, c_int, c_int, c_double_p,
+     c_int, c_double])
+delete_emissions = _export("delete_emissions", None, [c_double_p_p, c_int])
+argweaver_assert_emit = _export(
+    "arghmm_assert_emit", c_bool,
+    [c_void_p, c_int, c_double_p, c_double, c_char_p_p, c_int, c_int])
+forward_alg = _export("forward_alg", None, [c_int, c_int, c_double_p_p, c_double_p_p, c_double_p_p])
+backward_alg = _export("backward_alg", None, [c_int, c_int, c_double_p_p, c_double_p_p, c_double_p_p])
+sample_hmm_posterior = _export(
+    "sample_hmm_posterior", None, [c_int, c_int, c_double_p_p, c_double_p_p, c_int_p])
+argweaver_est_popsizes_trees = _export(
+    "arghmm_est_popsizes_trees", None, [c_void_p, c_double_p, c_int, c_int, c_double_p])
+
+# on-disk formats (argweaverc.py:31-38, 316-324)
+read_local_trees = _export("read_local_trees", c_void_p, [c_char_p, c_double_p, c_int])
+write_local_trees = _export(
+    "write_local_trees", None, [c_char_p, c_void_p, POINTER(c_char_p), c_double_p, c_int])
+argweaver_read_sites = _export("arghmm_read_sites", c_void_p, [c_char_p, c_int, c_int])
+argweaver_delete_sites = _export("arghmm_delete_sites", None, [c_void_p])
+awamd_sites_shape = _export("awamd_sites_shape", c_int, [c_void_p, c_int_p])
+awamd_sites_name = _export("awamd_sites_name", c_char_p, [c_void_p, c_int])
+awamd_sites_data = _export("awamd_sites_data", c_int, [c_void_p, c_int_p, c_char_p])
+awamd_sites_sequences = _export("awamd_sites_sequences", c_int, [c_void_p, ctypes.c_char, c_char_p])
+awamd_sites_compress = _export("awamd_sites_compress", c_void_p, [c_void_p, c_int])
+awamd_sites_uncompress = _export("awamd_sites_uncompress", c_int, [c_void_p, c_void_p])
+awamd_compress_local_trees = _export("awamd_compress_local_trees", c_int, [c_void_p, c_void_p])
+awamd_uncompress_local_trees = _export("awamd_uncompress_local_trees", c_int, [c_void_p, c_void_p])
+awamd_delete_sites_mapping = _export("awamd_delete_sites_mapping", None, [c_void_p])
+
 # extensions of this library
 awamd_remove_thread = _export("awamd_remove_thread", c_void_p, [c_void_p, c_int])
 awamd_thread_forward = _export(
@@ -470,6 +518,233 @@ def get_time_points(ntimes=20, maxtime=200e3, delta=.01):
     """Time grid (argweaver/__init__.py:97-105, model.h:21-32)."""
     return [(np.exp(i / float(ntimes - 1) * np.log(1 + delta * maxtime)) - 1) / delta
             for i in range(ntimes)]
+
+
+#=============================================================================
+# dense matrices, checks, generic HMM and file formats
+
+class DoubleMatrix(object):
+    """A double** view over a C-contiguous 2-D float64 array (kept alive here)."""
+
+    def __init__(self, arr):
+        self.arr = np.ascontiguousarray(arr, dtype=np.float64)
+        n = self.arr.shape[0]
+        addr = np.zeros(max(n, 1), np.uint64)
+        if n:
+            addr[:n] = self.arr.ctypes.data + np.arange(n, dtype=np.uint64) * self.arr.strides[0]
+        self.addr = addr
+        self.rows = (c_double_p * max(n, 1)).from_buffer(addr)
+
+    @property
+    def ptr(self):
+        return ctypes.cast(self.rows, c_double_p_p)
+
+
+def _rows_to_array(m, nrows, ncols):
+    out = np.zeros((nrows, ncols), np.float64)
+    for i in range(nrows):
+        out[i] = np.ctypeslib.as_array(m[i], shape=(ncols,))
+    return out
+
+
+def _states_arg(states):
+    st = np.ascontiguousarray(states, np.int32).reshape(-1, 2)
+    return st, st.ctypes.data_as(POINTER(intstate))
+
+
+def tree_states(ptree, ages, ntimes):
+    """States of one tree in node-major order (get_coal_states, states.cpp:53-75)."""
+    out = []
+    for i, p in enumerate(ptree):
+        top = ntimes - 2 if p == -1 else ages[p]
+        out.extend((i, a) for a in range(ages[i], top + 1))
+    return np.array(out, np.int32).reshape(-1, 2)
+
+
+def transition_probs(ptree, ages, times, popsizes, rho, states=None):
+    """log P(i -> j) within a block (new_transition_probs) as [S, S]."""
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    st, stp = _states_arg(tree_states(ptree, ages, len(times)) if states is None else states)
+    S = len(st)
+    z = np.zeros(len(times), np.int32)
+    m = new_transition_probs(len(ptree), _iptr(ptree), _iptr(ages), 0.0, stp, S, len(times),
+                             _dptr(times), _dptr(times), _iptr(z), _iptr(z), _iptr(z),
+                             _dptr(popsizes), rho)
+    if not m:
+        raise RuntimeError("new_transition_probs failed")
+    out = _rows_to_array(m, S, S)
+    delete_transition_probs(m, S)
+    return out
+
+
+def transition_probs_switch(ptree, last_ptree, ages, last_ages, spr, times, popsizes, rho):
+    """log P(i -> j) across an SPR (new_transition_probs_switch) as [S1, S2];
+    spr = (recomb_node, recomb_time, coal_node, coal_time)."""
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    last_ptree = np.ascontiguousarray(last_ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    last_ages = np.ascontiguousarray(last_ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    s1, s1p = _states_arg(tree_states(last_ptree, last_ages, len(times)))
+    s2, s2p = _states_arg(tree_states(ptree, ages, len(times)))
+    z = np.zeros(len(times), np.int32)
+    m = new_transition_probs_switch(
+        _iptr(ptree), _iptr(last_ptree), len(ptree), int(spr[0]), int(spr[1]), int(spr[2]),
+        int(spr[3]), _iptr(ages), _iptr(last_ages), 0.0, 0.0, s1p, len(s1), s2p, len(s2),
+        len(times), _dptr(times), _dptr(times), _iptr(z), _iptr(z), _iptr(z), _dptr(popsizes), rho)
+    if not m:
+        raise RuntimeError("new_transition_probs_switch failed")
+    out = _rows_to_array(m, len(s1), len(s2))
+    delete_transition_probs(m, len(s1))
+    return out
+
+
+def assert_transmat(ptree, ages, times, popsizes, rho):
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return bool(argweaver_assert_transmat(len(ptree), _iptr(ptree), _iptr(ages), len(times),
+                                          _dptr(times), _dptr(popsizes), rho))
+
+
+def assert_transmat_switch(ptree, ages, spr, times, popsizes, rho):
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return bool(argweaver_assert_transmat_switch(
+        len(ptree), _iptr(ptree), _iptr(ages), int(spr[0]), int(spr[1]), int(spr[2]), int(spr[3]),
+        len(times), _dptr(times), _dptr(popsizes), rho))
+
+
+def emissions(ptree, ages, seqs, times, mu, states=None):
+    """emit[column, state] of the new sequence seqs[nleaves] (new_emissions)."""
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    st, stp = _states_arg(tree_states(ptree, ages, len(times)) if states is None else states)
+    m = new_emissions(stp, len(st), _iptr(ptree), len(ptree), _iptr(ages), cs.ptr, cs.nseqs,
+                      cs.seqlen, _dptr(times), len(times), mu)
+    if not m:
+        raise RuntimeError("new_emissions failed")
+    out = _rows_to_array(m, cs.seqlen, len(st))
+    delete_emissions(m, cs.seqlen)
+    return out
+
+
+def assert_emit(trees, seqs, times, mu):
+    times = np.ascontiguousarray(times, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return bool(argweaver_assert_emit(trees, len(times), _dptr(times), mu, cs.ptr, cs.nseqs,
+                                      cs.seqlen))
+
+
+def hmm_forward(trans, emit, first):
+    """forward_alg (hmm.cpp:27-44) in log space; returns [n, S]."""
+    emit = np.ascontiguousarray(emit, np.float64)
+    n, S = emit.shape
+    fw = np.zeros((n, S), np.float64)
+    fw[0] = first
+    T, E, F = DoubleMatrix(trans), DoubleMatrix(emit), DoubleMatrix(fw)
+    forward_alg(n, S, T.ptr, E.ptr, F.ptr)
+    return F.arr
+
+
+def hmm_backward(trans, emit, last=None):
+    emit = np.ascontiguousarray(emit, np.float64)
+    n, S = emit.shape
+    bw = np.zeros((n, S), np.float64)
+    if last is not None:
+        bw[-1] = last
+    T, E, B = DoubleMatrix(trans), DoubleMatrix(emit), DoubleMatrix(bw)
+    backward_alg(n, S, T.ptr, E.ptr, B.ptr)
+    return B.arr
+
+
+def hmm_sample_posterior(trans, fw, last_state):
+    fw = np.ascontiguousarray(fw, np.float64)
+    n = fw.shape[0]
+    path = np.zeros(n, np.int32)
+    path[-1] = last_state
+    T, F = DoubleMatrix(trans), DoubleMatrix(fw)
+    sample_hmm_posterior(n, fw.shape[1], T.ptr, F.ptr, _iptr(path))
+    return path
+
+
+def est_popsizes_trees(trees, times, step):
+    times = np.ascontiguousarray(times, np.float64)
+    out = np.zeros(len(times), np.float64)
+    argweaver_est_popsizes_trees(trees, _dptr(times), len(times), step, _dptr(out))
+    return out
+
+
+def read_trees(filename, times):
+    """read_local_trees: an ARG handle from a .smc (or .smc.gz) file."""
+    times = np.ascontiguousarray(times, np.float64)
+    tr = read_local_trees(filename.encode(), _dptr(times), len(times))
+    if not tr:
+        raise IOError("cannot read local trees from %s" % filename)
+    return tr
+
+
+def write_trees(filename, trees, times, names=None):
+    times = np.ascontiguousarray(times, np.float64)
+    arr = None
+    if names is not None:
+        arr = (c_char_p * len(names))(*[n.encode() for n in names])
+    write_local_trees(filename.encode(), trees, arr, _dptr(times), len(times))
+
+
+class Sites(object):
+    """A .sites alignment (arghmm_read_sites) with its compression."""
+
+    def __init__(self, filename, start=-1, end=-1):
+        self.ptr = argweaver_read_sites(filename.encode(), start, end)
+        if not self.ptr:
+            raise IOError("cannot read sites from %s" % filename)
+
+    def shape(self):
+        sh = np.zeros(4, np.int32)
+        awamd_sites_shape(self.ptr, _iptr(sh))
+        return tuple(int(x) for x in sh)   # nseqs, nsites, start, end
+
+    @property
+    def names(self):
+        return [awamd_sites_name(self.ptr, i).decode() for i in range(self.shape()[0])]
+
+    def data(self):
+        nseqs, nsites, _, _ = self.shape()
+        pos = np.zeros(max(nsites, 1), np.int32)
+        cols = np.zeros((max(nsites, 1), nseqs), np.uint8)
+        awamd_sites_data(self.ptr, _iptr(pos), cols.ctypes.data_as(c_char_p))
+        return pos[:nsites], cols[:nsites]
+
+    def sequences(self, default="A"):
+        nseqs, _, start, end = self.shape()
+        out = np.zeros((nseqs, end - start), np.uint8)
+        awamd_sites_sequences(self.ptr, default.encode(), out.ctypes.data_as(c_char_p))
+        return out
+
+    def compress(self, factor):
+        """Compress in place; returns the mapping handle (SitesMapping)."""
+        m = awamd_sites_compress(self.ptr, factor)
+        if not m:
+            raise ValueError("sites cannot be compressed by %d" % factor)
+        return m
+
+    def uncompress(self, mapping):
+        awamd_sites_uncompress(self.ptr, mapping)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            argweaver_delete_sites(self.ptr)
+            self.ptr = None
 
 
 class Batch(object):

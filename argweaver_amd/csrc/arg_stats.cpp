@@ -10,31 +10,11 @@
 
 #include "../../include/argweaver_amd.h"
 #include "aw_math.h"
+#include "host_util.h"
 #include "local_trees.h"
 
 namespace aw {
 namespace {
-
-// flat arrays of one local tree, viewed as a TreeView
-struct FlatTree {
-    std::vector<int> par, c0, c1, age;
-    TreeView view;
-    explicit FlatTree(const LTree &t)
-    {
-        const int N = t.nnodes;
-        par.resize(N);
-        c0.resize(N);
-        c1.resize(N);
-        age.resize(N);
-        for (int i = 0; i < N; i++) {
-            par[i] = t.nodes[i].parent;
-            c0[i] = t.nodes[i].child[0];
-            c1[i] = t.nodes[i].child[1];
-            age[i] = t.nodes[i].age;
-        }
-        view = TreeView{N, t.root, par.data(), c0.data(), c1.data(), age.data()};
-    }
-};
 
 // children before parents (a node's partials depend on its children's
 // only, so every such order gives the same values)
@@ -120,6 +100,8 @@ double prob_coal_counts(int a, int b, double t, double n)
     return s;
 }
 
+}  // namespace
+
 // calc_spr_prob (total_prob.cpp:213-257): log probability of the SPR that
 // leads out of tree t (recombination point, then re-coalescence)
 double spr_logprob(const TreeView &t, const Model &m, const Spr &spr, double treelen)
@@ -146,6 +128,8 @@ double spr_logprob(const TreeView &t, const Model &m, const Spr &spr, double tre
     for (int x = 2 * k; x < 2 * j - 1; x++) lnl -= rates[x];
     return lnl;
 }
+
+namespace {
 
 // calc_arg_prior (total_prob.cpp:262-299)
 double arg_logprior(const ArgTrees &tr, const Model &m)

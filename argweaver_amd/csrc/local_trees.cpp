@@ -218,7 +218,7 @@ void remove_tree_branch(LTree &t, int remove_leaf, int *displace)
 }
 
 // add_spr_branch (thread.cpp:176-266)
-static void add_spr_branch(const LTree &t, const LTree &last, int st_node, int st_time,
+void add_spr_branch(const LTree &t, const LTree &last, int st_node, int st_time,
                            int last_node, Spr &spr, int *mapping, int newleaf, int displaced,
                            int newcoal)
 {

@@ -8,6 +8,7 @@
 //   add/remove leaf src/argweaver/thread.cpp:9-554
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "aw_math.h"
@@ -46,6 +47,7 @@ struct LBlock {
 
 struct ArgTrees {
     int start_coord = 0, end_coord = 0, nnodes = 0;
+    std::string chrom = "chr";  // chromosome name of the region (.smc REGION line)
     std::vector<LBlock> blocks;
     std::vector<int> seqids;    // leaf -> sequence row
 
@@ -65,6 +67,11 @@ void tree_from_parents(LTree &t, const int *ptree, const int *ages, int nnodes, 
 void apply_spr(LTree &t, const Spr &spr);
 void add_tree_branch(LTree &t, int node, int time);
 void remove_tree_branch(LTree &t, int remove_leaf, int *displace);
+
+// add_spr_branch (thread.cpp:176-266): rewrite the SPR into tree t (both
+// trees carrying the new branch) and extend the node mapping for it
+void add_spr_branch(const LTree &t, const LTree &last, int st_node, int st_time, int last_node,
+                    Spr &spr, int *mapping, int newleaf, int displaced, int newcoal);
 
 // Splice a sampled thread into the ARG (thread.cpp:272-418).  path holds the
 // state index of every column (relative to start_coord); recomb_pos are

@@ -120,8 +120,93 @@ double arghmm_joint_prob(LocalTrees *trees, double *times, int ntimes, double *p
                          double mu, double rho, char **seqs, int nseqs, int seqlen);
 
 /*---------------------------------------------------------------------------
+ * dense matrices, their brute-force checks and the generic HMM (host; not on
+ * the threading path, kept so the C ABI is complete).  Matrices are arrays
+ * of separately allocated rows, freed by the matching delete_* call.
+ */
+/* src/argweaver/trans.cpp:1190-1208: log P(i -> j) within a block; the tree
+ * length, time steps and lineage counts are recomputed from the tree */
+double **new_transition_probs(int nnodes, int *ptree, int *ages, double treelen,
+                              intstate *states, int nstates, int ntimes, double *times,
+                              double *time_steps, int *nbranches, int *nrecombs, int *ncoals,
+                              double *popsizes, double rho);
+/* src/argweaver/trans.cpp:1211-1244: log P(i -> j) across the SPR between
+ * last_ptree and ptree (states of the two trees in node-major order) */
+double **new_transition_probs_switch(int *ptree, int *last_ptree, int nnodes, int recomb_name,
+                                     int recomb_time, int coal_name, int coal_time,
+                                     int *ages_index, int *last_ages_index, double treelen,
+                                     double last_treelen, intstate *states1, int nstates1,
+                                     intstate *states2, int nstates2, int ntimes, double *times,
+                                     double *time_steps, int *nbranches, int *nrecombs,
+                                     int *ncoals, double *popsizes, double rho);
+/* src/argweaver/trans.cpp:1248-1251 */
+void delete_transition_probs(double **transmat, int nstates);
+/* src/argweaver/trans.cpp:1254-1271 (assert_transmat, :856-943) */
+bool arghmm_assert_transmat(int nnodes, int *ptree, int *ages, int ntimes, double *times,
+                            double *popsizes, double rho);
+/* src/argweaver/trans.cpp:1275-1315 (assert_transmat_switch, :947-1032) */
+bool arghmm_assert_transmat_switch(int nnodes, int *ptree, int *ages, int recomb_node,
+                                   int recomb_time, int coal_node, int coal_time, int ntimes,
+                                   double *times, double *popsizes, double rho);
+/* src/argweaver/emit.cpp:1288-1303: emit[seqlen][nstates] for the new
+ * sequence seqs[nleaves] against one tree */
+double **new_emissions(intstate *states, int nstates, int *ptree, int nnodes, int *ages_index,
+                       char **seqs, int nseqs, int seqlen, double *times, int ntimes, double mu);
+/* src/argweaver/emit.cpp:1306-1309 */
+void delete_emissions(double **emit, int seqlen);
+/* src/argweaver/emit.cpp:1312-1337 (assert_emissions, :1222-1250) */
+bool arghmm_assert_emit(LocalTrees *trees, int ntimes, double *times, double mu, char **seqs,
+                        int nseqs, int seqlen);
+/* src/argweaver/hmm.cpp:27-44: log-space forward; fw[0] is the caller's */
+void forward_alg(int n, int nstates, double **trans, double **emit, double **fw);
+/* src/argweaver/hmm.cpp:48-64: log-space backward; bw[n-1] is the caller's */
+void backward_alg(int n, int nstates, double **trans, double **emit, double **bw);
+/* src/argweaver/hmm.cpp:67-84: path[n-1] is the caller's; draws rand() */
+void sample_hmm_posterior(int n, int nstates, double **trans, double **fw, int *path);
+/* src/argweaver/est_popsize.cpp:140-147: one tree every `step` bases;
+ * popsizes[0..ntimes-2] */
+void arghmm_est_popsizes_trees(LocalTrees *trees, double *times, int ntimes, int step,
+                               double *popsizes);
+
+/*---------------------------------------------------------------------------
+ * on-disk formats (host)
+ */
+/* src/argweaver/local_tree.cpp:1904-1931: .smc (plain or gzip); NULL on a
+ * malformed file */
+LocalTrees *read_local_trees(const char *filename, const double *times, int ntimes);
+/* src/argweaver/local_tree.cpp:1897-1901 (writer :1286-1341); names may be
+ * NULL (no NAMES line) */
+void write_local_trees(char *filename, LocalTrees *trees, char **names, double *times,
+                       int ntimes);
+/* src/argweaver/sequences.cpp:783-793: .sites alignment (plain or gzip);
+ * subregion bounds of -1 keep the file's REGION; NULL on a malformed file */
+void *arghmm_read_sites(const char *filename, int subregion_start, int subregion_end);
+/* src/argweaver/sequences.cpp:796-799 */
+void arghmm_delete_sites(void *sites);
+
+/*---------------------------------------------------------------------------
  * extensions (no counterpart in the reference)
  */
+/* A sites handle: shape4 = {nseqs, nsites, start_coord, end_coord}; the
+ * name of sequence i; positions[nsites] and the columns as nsites rows of
+ * nseqs bases (either may be NULL); the alignment as nseqs rows of
+ * end_coord - start_coord bases with default_char off the variant sites
+ * (make_sequences_from_sites, sequences.cpp:315-343). */
+int awamd_sites_shape(void *sites, int *shape4);
+const char *awamd_sites_name(void *sites, int i);
+int awamd_sites_data(void *sites, int *positions, char *cols);
+int awamd_sites_sequences(void *sites, char default_char, char *seqs);
+/* Compression (find_compress_cols + compress_sites, sequences.cpp:523-611):
+ * maps every `compress` bases to one column, keeping every variant site, in
+ * place; returns the mapping (NULL when not possible), which also
+ * (un)compresses the block lengths of local trees (local_tree.cpp:790-848)
+ * and restores the sites. */
+void *awamd_sites_compress(void *sites, int compress);
+int awamd_sites_uncompress(void *sites, void *mapping);
+int awamd_compress_local_trees(LocalTrees *trees, void *mapping);
+int awamd_uncompress_local_trees(LocalTrees *trees, void *mapping);
+void awamd_delete_sites_mapping(void *mapping);
+
 /* Remove the leaf holding sequence `seqid` from the ARG (the step
  * resample_arg performs before re-threading). */
 LocalTrees *awamd_remove_thread(LocalTrees *trees, int seqid);

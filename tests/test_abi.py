@@ -50,3 +50,28 @@ def test_tree_arrays_round_trip_and_block_extent():
         assert np.array_equal(x, y)
     A.delete_local_trees(tr2)
     A.delete_local_trees(tr)
+
+
+# internal-branch resampling entry points of the reference's binding that
+# this library does not export yet
+NOT_YET_EXPORTED = {
+    "arghmm_assert_emit_internal", "arghmm_assert_transmat_internal",
+    "arghmm_assert_transmat_switch_internal", "arghmm_get_thread_times",
+    "arghmm_remove_arg_thread_path", "arghmm_remove_arg_thread_path2",
+    "arghmm_resample_all_arg", "arghmm_resample_arg_region", "arghmm_resample_climb_arg",
+    "arghmm_resample_mcmc_arg", "arghmm_sample_arg_removal_leaf_path",
+    "arghmm_sample_arg_removal_path", "arghmm_sample_arg_removal_path2",
+    "arghmm_sample_arg_removal_path_recomb", "arghmm_sample_arg_thread_internal",
+}
+
+
+def test_reference_binding_names_exported():
+    """Every C entry point the reference's ctypes binding binds by name
+    (argweaver/argweaverc.py:29-343, listed in tests/golden/
+    argweaverc_exports.txt) is exported, save the listed internal-branch ones."""
+    from argweaver_amd import load_library
+    lib = load_library()
+    names = open(os.path.join(ROOT, "tests", "golden", "argweaverc_exports.txt")).read().split()
+    assert len(names) == 57
+    missing = {n for n in names if not hasattr(lib, n)}
+    assert missing == NOT_YET_EXPORTED
