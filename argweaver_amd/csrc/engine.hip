@@ -914,6 +914,12 @@ __device__ __forceinline__ void vrow_load(const DevInst &d, int v, double (&ev)[
     for (int r = 0; r < R; r++) ev[r] = row[r * 64 + threadIdx.x];
 }
 
+// 1 / x: FP32 tables use the hardware reciprocal (within 1 ulp; the
+// normaliser itself, and so the log-likelihood, is kept exactly), FP64
+// tables the correctly rounded division
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 template <class Real>
 __device__ __forceinline__ void lds_add(Real *p, Real v)
 {
@@ -1006,6 +1012,9 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
     double *GP = (double *)sm;       // 32 x GCOPY: per-time partial sums
     // byte offset of this lane's copy within a time bin's GCOPY copies
     const int gslot = (lane / (64 / GCOPY)) * (int)sizeof(double);
+    // the half of bin lane>>1 that this lane reduces, and its read rotation
+    double *gq = GP + (lane >> 1) * GCOPY + (lane & 1) * (GCOPY / 2);
+    const int grot = (lane >> 1) & (GCOPY / 2 - 1);
     Real *XS = (Real *)(GP + 32 * GCOPY);   // RW: previous column in state order (switch)
     Real *NW = XS + RW;              // RW + 64: switch targets, then a dummy slot per lane
     Real *VS = NW + RW + 64;         // VST rows: staged variant-column emissions
@@ -1050,14 +1059,17 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
 #pragma unroll
         for (int r = 0; r < R; r++)
             lds_add((double *)((char *)GP + fl.boff[r] * (2 * GCOPY) + gslot), (double)v[r]);
-        // lane l adds the copies of bin l>>1 in its half (l&1) and clears them
-        // (the LDS operations of one wave run in order)
-        double *gq = GP + (lane >> 1) * GCOPY + (lane & 1) * (GCOPY / 2);
+        // lane l takes the copies of bin l>>1 in its half (l&1), leaving 0
+        // (one exchange each; the LDS operations of one wave run in order).
+        // Step k reads copy (k + l/2) mod 8 of the half: the lanes of one
+        // step then spread over the banks (copy k of every bin would sit in
+        // two banks), which also fixes each lane's summation order
         double q[GCOPY / 2];
 #pragma unroll
-        for (int k = 0; k < GCOPY / 2; k++) q[k] = gq[k];
-#pragma unroll
-        for (int k = 0; k < GCOPY / 2; k++) gq[k] = 0.0;
+        for (int k = 0; k < GCOPY / 2; k++)
+            q[k] = __longlong_as_double((long long)__hip_atomic_exchange(
+                (unsigned long long *)(gq + ((k + grot) & (GCOPY / 2 - 1))), 0ull,
+                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
 #pragma unroll
         for (int w = 1; w < GCOPY / 2; w *= 2)
 #pragma unroll
@@ -1076,7 +1088,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
         const Real t = (tq[0] + tq[1]) + (tq[2] + tq[3]);
         // column 0 is stored unnormalised (scale 1); a restarted tile's first
         // column is the boundary column as stored by the tile before
-        const Real nsc = (col == 0 || (first && tile.mode == 2)) ? Real(1) : Real(1) / norm;
+        const Real nsc = (col == 0 || (first && tile.mode == 2)) ? Real(1) : recip(norm);
         if (col >= cs && col < tile.c1) {
             AW_GLOBAL Real *row = alpha + (size_t)col * RW;
             if (tile.mode == 2 && !first && (col - tile.c0) % FW_RECHECK == 0) {
