@@ -15,6 +15,7 @@
 #include "engine.h"
 #include "serve.h"
 #include "local_trees.h"
+#include "thread_path.h"
 
 namespace aw {
 
@@ -277,15 +278,12 @@ void delete_local_trees(LocalTrees *trees) { delete reinterpret_cast<ArgTrees *>
 
 void arghmm_get_nstates(LocalTrees *trees, int ntimes, bool internal, int *nstates)
 {
-    if (internal) {
-        fprintf(stderr, "argweaver_amd: internal-branch state spaces are not supported\n");
-        abort();
-    }
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
     std::vector<int> sn, st;
     int i = 0;
     for (const LBlock &B : tr.blocks) {
-        tree_states(B.tree, ntimes, sn, st);
+        if (internal) internal_states(B.tree, ntimes, 0, sn, st);
+        else tree_states(B.tree, ntimes, sn, st);
         for (int j = 0; j < B.blocklen; j++) nstates[i + j] = (int)sn.size();
         i += B.blocklen;
     }

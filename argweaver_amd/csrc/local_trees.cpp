@@ -376,7 +376,7 @@ void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
 }
 
 // remove_null_spr / remove_null_sprs (local_tree.cpp:409-469)
-static void remove_null_sprs(ArgTrees &tr)
+void remove_null_sprs(ArgTrees &tr)
 {
     std::vector<LBlock> out;
     out.reserve(tr.blocks.size());

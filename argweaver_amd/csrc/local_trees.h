@@ -82,6 +82,10 @@ void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
 // Remove the leaf holding sequence `seqid` (thread.cpp:425-554).
 void remove_arg_thread(ArgTrees &tr, int seqid);
 
+// Merge every block whose SPR is null into the block before it
+// (remove_null_sprs, local_tree.cpp:409-461).
+void remove_null_sprs(ArgTrees &tr);
+
 // Structural checks run after every edit (assert_trees, local_tree.cpp:1763-1809).
 bool check_trees(const ArgTrees &tr);
 

@@ -169,6 +169,32 @@ void arghmm_est_popsizes_trees(LocalTrees *trees, double *times, int ntimes, int
                                double *popsizes);
 
 /*---------------------------------------------------------------------------
+ * internal-branch re-threading: removal paths and the branch surgery (host)
+ */
+/* src/argweaver/thread.cpp:1681-1684: a removal path (one node per local
+ * tree) from `node` of the first tree, forward through the SPRs */
+void arghmm_sample_arg_removal_path(LocalTrees *trees, int node, int *path);
+/* src/argweaver/thread.cpp:1686-1690: a removal path through `node` of the
+ * tree at position pos, forward and backward */
+void arghmm_sample_arg_removal_path2(LocalTrees *trees, int node, int pos, int *path);
+/* src/argweaver/thread.cpp:1692-1696: the path of a leaf */
+void arghmm_sample_arg_removal_leaf_path(LocalTrees *trees, int node, int *path);
+/* src/argweaver/thread.cpp:1698-1703: preferring branches that carry
+ * recombinations */
+void arghmm_sample_arg_removal_path_recomb(LocalTrees *trees, double recomb_preference,
+                                           int *path);
+/* src/argweaver/thread.cpp:1705-1709: remove the branch along the path,
+ * leaving each tree in subtree/maintree form under a root of age maxtime */
+void arghmm_remove_arg_thread_path(LocalTrees *trees, int *removal_path, int maxtime);
+/* src/argweaver/thread.cpp:1711-1716: as above, with the removed branch as
+ * internal state indices per column */
+void arghmm_remove_arg_thread_path2(LocalTrees *trees, int *removal_path, int maxtime,
+                                    int *original_thread);
+/* src/argweaver/thread.cpp:1719-1735: the time of each column's internal
+ * state */
+void arghmm_get_thread_times(LocalTrees *trees, int ntimes, int *path, int *path_times);
+
+/*---------------------------------------------------------------------------
  * on-disk formats (host)
  */
 /* src/argweaver/local_tree.cpp:1904-1931: .smc (plain or gzip); NULL on a
