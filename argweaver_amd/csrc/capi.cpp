@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/argweaver_amd.h"
@@ -16,6 +17,7 @@
 #include "serve.h"
 #include "local_trees.h"
 #include "thread_path.h"
+#include "internal_prep.h"
 
 namespace aw {
 
@@ -173,6 +175,57 @@ static int sample_arg_thread(ArgTrees &tr, const double *times, int ntimes, cons
 
 // false (with a message) when no HIP device is visible: the HMM entry
 // points then return NULL instead of computing anything on the host
+// The internal-branch instance of an ARG whose trees are in
+// subtree/maintree form: per-block inputs prepared on the host
+// (prepare_internal), forward and traceback on the device.
+static int build_internal(const ArgTrees &tr, const double *times, int ntimes,
+                          const double *popsizes, double rho, double mu, char **seqs, int minage,
+                          HostInstance &hi)
+{
+    if (ntimes < 2 || ntimes > MAXT) {
+        fprintf(stderr, "argweaver_amd: %d time points (supported: 2..%d)\n", ntimes, MAXT);
+        return 1;
+    }
+    flatten(tr, hi.ft);
+    hi.rows.clear();   // the device kernels read the prepared inputs only
+    std::vector<double> pop(ntimes);
+    for (int i = 0; i < ntimes; i++) pop[i] = popsizes ? popsizes[i] : 0.0;
+    model_init(hi.model, ntimes, times, pop.data(), rho, mu);
+    auto hp = std::make_shared<HostPrep>();
+    if (prepare_internal(tr, hi.model, seqs, minage, *hp)) return 1;
+    hi.prep = hp;
+    return 0;
+}
+
+// forward and traceback of the internal branch (sample_thread.cpp:637-668,
+// 981-1006): one uniform per column, last column first
+static int internal_thread_path(const ArgTrees &tr, const double *times, int ntimes,
+                                const double *popsizes, double rho, double mu, char **seqs,
+                                int minage, std::vector<int> &path, HostInstance &hi)
+{
+    if (build_internal(tr, times, ntimes, popsizes, rho, mu, seqs, minage, hi)) return 1;
+    const int L = tr.length();
+    std::vector<double> u(L);
+    for (int i = 0; i < L; i++) u[i] = frand();
+    path.assign(L, 0);
+    return run_instance(hi, u.data(), path.data(), nullptr, nullptr, nullptr);
+}
+
+// sample_arg_thread_internal (sample_thread.cpp:637-694): re-thread the
+// removed subtree into the ARG
+static int sample_arg_thread_internal(ArgTrees &tr, const double *times, int ntimes,
+                                      const double *popsizes, double rho, double mu, char **seqs,
+                                      int minage)
+{
+    HostInstance hi;
+    std::vector<int> path;
+    if (internal_thread_path(tr, times, ntimes, popsizes, rho, mu, seqs, minage, path, hi))
+        return 1;
+    std::vector<int> rpos, rnode, rtime;
+    sample_recombinations_internal(tr, hi.model, minage, path.data(), rpos, rnode, rtime);
+    return add_thread_path(tr, ntimes, minage, path.data(), rpos, rnode, rtime);
+}
+
 static bool require_gpu(const char *fn)
 {
     if (device_available()) return true;
@@ -320,21 +373,28 @@ double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double
                             bool prior_given, double *prior, bool internal, bool slow)
 {
     (void)slow;   // the dense variant computes the same table; one device kernel serves both
-    if (internal) {
-        fprintf(stderr, "argweaver_amd: internal-branch forward is not supported\n");
-        return nullptr;
-    }
     if (!require_gpu("arghmm_forward_alg")) return nullptr;
     ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
     HostInstance hi;
-    const int new_chrom = tr.num_leaves();   // ArgHmmMatrixIter default (matrices.h:224-226)
-    if (new_chrom >= nseqs) {
-        fprintf(stderr, "argweaver_amd: arghmm_forward_alg needs %d sequences\n", new_chrom + 1);
-        return nullptr;
-    }
-    if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi)) return nullptr;
     std::vector<int> Sv;
-    long long W = instance_states(hi.ft, ntimes, &Sv);
+    long long W = 0;
+    if (internal) {
+        // the trees are in subtree/maintree form; the subtree is the thread
+        if (build_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0, hi)) return nullptr;
+        Sv = hi.prep->S;
+        for (int b = 0; b < hi.ft.nblocks; b++) W += (long long)Sv[b] * hi.ft.len[b];
+        prior_given = false;   // the prepared first column is the reference's internal prior
+    } else {
+        const int new_chrom = tr.num_leaves();   // ArgHmmMatrixIter default (matrices.h:224-226)
+        if (new_chrom >= nseqs) {
+            fprintf(stderr, "argweaver_amd: arghmm_forward_alg needs %d sequences\n",
+                    new_chrom + 1);
+            return nullptr;
+        }
+        if (build_instance(tr, times, ntimes, popsizes, rho, mu, seqs, new_chrom, hi))
+            return nullptr;
+        W = instance_states(hi.ft, ntimes, &Sv);
+    }
     double *table = new double[W > 0 ? W : 1];
     if (run_instance(hi, nullptr, nullptr, table, nullptr, nullptr, Precision::FP64,
                      prior_given ? prior : nullptr)) {
@@ -511,6 +571,101 @@ LocalTrees *arghmm_resample_arg(LocalTrees *trees, double *times, int ntimes, do
         if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) return nullptr;
     for (int i = 0; i < niters; i++)
         if (resample_arg(tr, times, ntimes, popsizes, rho, mu, seqs, nremove)) return nullptr;
+    return trees;
+}
+
+void arghmm_sample_arg_thread_internal(LocalTrees *trees, double *times, int ntimes,
+                                       double *popsizes, double rho, double mu, char **seqs,
+                                       int nseqs, int seqlen, int *thread_path)
+{
+    (void)nseqs; (void)seqlen;
+    if (!require_gpu("arghmm_sample_arg_thread_internal")) return;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    HostInstance hi;
+    std::vector<int> path;
+    if (internal_thread_path(tr, times, ntimes, popsizes, rho, mu, seqs, 0, path, hi)) {
+        fprintf(stderr, "argweaver_amd: arghmm_sample_arg_thread_internal failed\n");
+        return;
+    }
+    std::copy(path.begin(), path.end(), thread_path);
+}
+
+// irand (common.h:109-113)
+static int irand_(int max)
+{
+    const int i = int(rand() / float(RAND_MAX) * max);
+    return (i == max) ? max - 1 : i;
+}
+static int irand_(int lo, int hi)
+{
+    const int i = lo + int(rand() / float(RAND_MAX) * (hi - lo));
+    return (i == hi) ? hi - 1 : i;
+}
+
+// resample_arg_all (sample_arg.cpp:111-127): a random branch at a random
+// position, its removal path, then the internal re-threading
+static int resample_arg_all(ArgTrees &tr, const double *times, int ntimes,
+                            const double *popsizes, double rho, double mu, char **seqs,
+                            double prob_path_switch)
+{
+    const int maxtime = ntimes + 1;
+    std::vector<int> path(tr.blocks.size());
+    const int node = irand_(tr.nnodes);
+    const int pos = irand_(tr.start_coord, tr.end_coord);
+    removal_path_at(tr, node, pos, path.data(), prob_path_switch);
+    remove_thread_path(tr, path.data(), maxtime, nullptr);
+    return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);
+}
+
+// resample_arg_recomb (sample_arg.cpp:211-223)
+static int resample_arg_recomb(ArgTrees &tr, const double *times, int ntimes,
+                               const double *popsizes, double rho, double mu, char **seqs,
+                               double recomb_preference)
+{
+    std::vector<int> path(tr.blocks.size());
+    removal_path_recomb(tr, recomb_preference, path.data());
+    remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
+    return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);
+}
+
+// resample_arg_leaf (sample_arg.cpp:131-149), phased data
+static int resample_arg_leaf(ArgTrees &tr, const double *times, int ntimes,
+                             const double *popsizes, double rho, double mu, char **seqs)
+{
+    std::vector<int> path(tr.blocks.size());
+    const int node = irand_(tr.num_leaves());
+    removal_leaf_path(tr, node, path.data());
+    remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
+    return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);
+}
+
+LocalTrees *arghmm_resample_all_arg(LocalTrees *trees, double *times, int ntimes,
+                                    double *popsizes, double rho, double mu, char **seqs,
+                                    int nseqs, int seqlen, int niters, double prob_path_switch)
+{
+    (void)seqlen;
+    if (!require_gpu("arghmm_resample_all_arg")) return nullptr;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    for (int c = tr.num_leaves(); c < nseqs; c++)
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) return nullptr;
+    for (int i = 0; i < niters; i++)
+        if (resample_arg_all(tr, times, ntimes, popsizes, rho, mu, seqs, prob_path_switch))
+            return nullptr;
+    return trees;
+}
+
+LocalTrees *arghmm_resample_climb_arg(LocalTrees *trees, double *times, int ntimes,
+                                      double *popsizes, double rho, double mu, char **seqs,
+                                      int nseqs, int seqlen, int niters, double recomb_preference)
+{
+    (void)seqlen;
+    if (!require_gpu("arghmm_resample_climb_arg")) return nullptr;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    for (int c = tr.num_leaves(); c < nseqs; c++)
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) return nullptr;
+    for (int i = 0; i < niters; i++)
+        if (resample_arg_recomb(tr, times, ntimes, popsizes, rho, mu, seqs, recomb_preference))
+            return nullptr;
     return trees;
 }
 

@@ -7,7 +7,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "aw_math.h"
+#include "internal_prep.h"
 #include "local_trees.h"
 
 namespace aw {
@@ -48,6 +51,7 @@ struct DevInst {
     double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef), FP64 runs
     float *lcoef32;           // [B*NCOEF*RW] the planes in FP32, FP32 runs
     int fp32;                 // the run is FP32
+    int hostprep;             // the per-block inputs come from the host (internal threading)
     int *ldet;                // [B*RW] deterministic switch target of block b-1's states
     double *lprior;           // [RW] first-column prior
     double *vemit;            // [nvar*RW] emissions of variant columns
@@ -110,6 +114,9 @@ struct HostInstance {
     FlatTrees ft;
     std::vector<const char *> rows;   // nleaves rows in leaf order, then the new sequence
     Model model;
+    // internal threading: the per-block inputs prepared on the host (the
+    // device preparation, classification and emission kernels skip it)
+    std::shared_ptr<const HostPrep> prep;
 };
 
 // Schedule of a batch run.  The forward recursion runs one wave per

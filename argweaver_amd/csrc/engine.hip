@@ -160,6 +160,7 @@ __global__ void k_classify_b(const DevInst *__restrict__ D, const long long *col
     if (g >= coloff[ninst]) return;
     int i = find_owner(coloff, ninst, g);
     const DevInst &d = D[i];
+    if (d.hostprep) return;
     int col = (int)(g - coloff[i]);
     char c0 = gp(d.seq)[col];
     bool inv = true;
@@ -176,6 +177,10 @@ __global__ __launch_bounds__(1024) void k_scan_b(const DevInst *__restrict__ D, 
     __shared__ int part[1024];
     const DevInst &d = D[blockIdx.x];
     const int tid = threadIdx.x;
+    if (d.hostprep) {   // the whole workgroup leaves: the host knows the count
+        if (tid == 0) nvar_out[blockIdx.x] = 0;
+        return;
+    }
     const int per = (d.L + 1023) / 1024;
     const int s0 = min(d.L, tid * per), e0 = min(d.L, s0 + per);
     int loc = 0;
@@ -316,6 +321,7 @@ __global__ __launch_bounds__(256) void k_prep_b(const DevInst *__restrict__ D,
     __shared__ PrepShared sh;
     const int inst = find_owner(blkoff, ninst, (int)blockIdx.x);
     const DevInst &d = D[inst];
+    if (d.hostprep) return;   // uniform over the workgroup
     const int b = blockIdx.x - blkoff[inst];
     const int tid = threadIdx.x, NT = blockDim.x;
     const int T = d.T, N = d.N, T1 = T - 1;
@@ -754,6 +760,7 @@ __global__ __launch_bounds__(64) void k_var_emit_b(const DevInst *__restrict__ D
     char *bases = (char *)(lvl + nmax);
     const int inst = find_owner(voff, ninst, (int)blockIdx.x);
     const DevInst &d = D[inst];
+    if (d.hostprep) return;   // uniform over the wave
     const int v = blockIdx.x - voff[inst];
     const int lane = threadIdx.x;
     const int N = d.N, R = d.R, RW = 64 * R;
@@ -1875,6 +1882,7 @@ struct Batch::Impl {
     std::vector<Chunk> chunks;
     DevBuf in, work, tab, aux, vbuf, tiles, fwb;
     Arena arena;   // base == nullptr: buffers come from hipMalloc
+    std::vector<std::vector<float>> staged32;   // FP32 planes of host-prepared instances
     Streams st;
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
@@ -1968,7 +1976,13 @@ int Batch::add(const HostInstance &hi, const double *u, const double *prior)
     if (u) m.us.back().assign(u, u + hi.ft.seqlen);
     m.priors.emplace_back();
     std::vector<int> Sv;
-    long long W = instance_states(hi.ft, hi.model.T, &Sv);
+    long long W = 0;
+    if (hi.prep) {
+        Sv = hi.prep->S;
+        for (int b = 0; b < hi.ft.nblocks; b++) W += (long long)Sv[b] * hi.ft.len[b];
+    } else {
+        W = instance_states(hi.ft, hi.model.T, &Sv);
+    }
     if (prior) m.priors.back().assign(prior, prior + Sv[0]);
     m.W.push_back(W);
     m.Sv.push_back(Sv);
@@ -2016,6 +2030,7 @@ int Batch::upload()
                             Smax);
                     return 1;
                 }
+                d.hostprep = hi.prep ? 1 : 0;
                 d.T = T; d.N = N; d.B = B; d.L = L; d.nrows = (int)hi.rows.size();
                 d.Smax = Smax; d.TC = tc_base(T - 1, T - 1); d.W = m.W[i]; d.R = R;
             }
@@ -2446,6 +2461,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(m.nvar.data(), m.d_nvar, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++)
+        if (m.hosts[i].prep) m.nvar[i] = m.hosts[i].prep->nvar;
 
     // variant rows, allocated now that the counts are known
     std::vector<int> voff(n + 1, 0);
@@ -2474,6 +2491,39 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         for (size_t k = 0; k < m.priors[i].size(); k++) lpri[i][lane_pos((int)k, R)] = m.priors[i][k];
         HIP_OK(hipMemcpyAsync(m.h[i].lprior, lpri[i].data(), sizeof(double) * lpri[i].size(),
                               hipMemcpyHostToDevice, s));
+    }
+    // host-prepared instances: their per-block inputs, column classes and
+    // variant rows (the device kernels above skipped them)
+    m.staged32.clear();
+    for (int i = 0; i < n; i++) {
+        const HostPrep *hp = m.hosts[i].prep.get();
+        if (!hp) continue;
+        const DevInst &d = m.h[i];
+        const int B = d.B, RW = 64 * d.R, T1 = d.T - 1;
+        if (fp32) {
+            m.staged32.emplace_back(hp->planes.begin(), hp->planes.end());
+            HIP_OK(hipMemcpyAsync(d.lcoef32, m.staged32.back().data(),
+                                  sizeof(float) * hp->planes.size(), hipMemcpyHostToDevice, s));
+        } else {
+            HIP_OK(hipMemcpyAsync(d.lcoef, hp->planes.data(), sizeof(double) * hp->planes.size(),
+                                  hipMemcpyHostToDevice, s));
+        }
+        HIP_OK(hipMemcpyAsync(d.linfo, hp->info.data(), sizeof(int) * (size_t)B * RW,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.ldet, hp->det.data(), sizeof(int) * (size_t)B * RW,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.tm, hp->tm.data(), sizeof(double) * (size_t)B * T1 * T1,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.tm2s, hp->tm2s.data(), sizeof(double) * (size_t)B * d.TC,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.srcs, hp->srcs.data(), sizeof(int) * 2 * (size_t)B,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.lprior, hp->prior.data(), sizeof(double) * RW,
+                              hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d.cls, hp->cls.data(), sizeof(int) * d.L, hipMemcpyHostToDevice, s));
+        if (hp->nvar > 0)
+            HIP_OK(hipMemcpyAsync(d.vemit, hp->vemit.data(), sizeof(double) * hp->vemit.size(),
+                                  hipMemcpyHostToDevice, s));
     }
     (void)hipEventRecord(c.ev[2], s);
     if (voff[n] > 0) {
@@ -2618,7 +2668,7 @@ int instance_supported(const HostInstance &hi, std::string *msg)
     if (ft.nnodes > MAXN || hi.model.T > MAXT) {
         snprintf(buf, sizeof buf, "instance has %d nodes and %d time points (limits %d and %d)",
                  ft.nnodes, hi.model.T, MAXN, MAXT);
-    } else {
+    } else if (!hi.prep) {
         std::vector<int> Sv;
         instance_states(ft, hi.model.T, &Sv);
         int Smax = 1;

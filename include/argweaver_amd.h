@@ -70,7 +70,9 @@ void delete_state_spaces(intstate **all_states, int ntrees);
  * threading HMM
  */
 /* src/argweaver/sample_thread.cpp:887-926.  Row-per-column table, freed with
- * delete_forward_matrix.  internal=true is unsupported (returns NULL). */
+ * delete_forward_matrix.  internal=true: the trees are in subtree/maintree
+ * form and the removed subtree is the thread (the caller's prior is not
+ * used then; the reference's internal prior is). */
 double **arghmm_forward_alg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
                             double rho, double mu, char **seqs, int nseqs, int seqlen,
                             bool prior_given, double *prior, bool internal, bool slow);
@@ -97,6 +99,23 @@ LocalTrees *arghmm_sample_arg_refine(double *times, int ntimes, double *popsizes
 LocalTrees *arghmm_resample_arg(LocalTrees *trees, double *times, int ntimes, double *popsizes,
                                 double rho, double mu, char **seqs, int nseqs, int seqlen,
                                 int niters, int nremove);
+/* src/argweaver/sample_thread.cpp:981-1006: forward and traceback of the
+ * subtree of an ARG whose trees are in subtree/maintree form (after
+ * arghmm_remove_arg_thread_path); thread_path receives one internal state
+ * index per column.  The ARG is not changed. */
+void arghmm_sample_arg_thread_internal(LocalTrees *trees, double *times, int ntimes,
+                                       double *popsizes, double rho, double mu, char **seqs,
+                                       int nseqs, int seqlen, int *thread_path);
+/* src/argweaver/sample_arg.cpp:722-739: niters re-threadings of a random
+ * internal branch (resample_arg_all, :111-127) */
+LocalTrees *arghmm_resample_all_arg(LocalTrees *trees, double *times, int ntimes,
+                                    double *popsizes, double rho, double mu, char **seqs,
+                                    int nseqs, int seqlen, int niters, double prob_path_switch);
+/* src/argweaver/sample_arg.cpp:790-806: niters re-threadings along removal
+ * paths weighted towards recombinations (resample_arg_climb, :211-230) */
+LocalTrees *arghmm_resample_climb_arg(LocalTrees *trees, double *times, int ntimes,
+                                      double *popsizes, double rho, double mu, char **seqs,
+                                      int nseqs, int seqlen, int niters, double recomb_preference);
 /* src/argweaver/sample_thread.cpp:1026-1029 */
 void delete_path(int *path);
 /* src/argweaver/logging.cpp:126-127 */

@@ -56,9 +56,8 @@ def test_tree_arrays_round_trip_and_block_extent():
 # this library does not export yet
 NOT_YET_EXPORTED = {
     "arghmm_assert_emit_internal", "arghmm_assert_transmat_internal",
-    "arghmm_assert_transmat_switch_internal", "arghmm_resample_all_arg",
-    "arghmm_resample_arg_region", "arghmm_resample_climb_arg", "arghmm_resample_mcmc_arg",
-    "arghmm_sample_arg_thread_internal",
+    "arghmm_assert_transmat_switch_internal",
+    "arghmm_resample_arg_region", "arghmm_resample_mcmc_arg",
 }
 
 
