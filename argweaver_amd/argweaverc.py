@@ -108,6 +108,12 @@ argweaver_assert_transmat = _export(
 argweaver_assert_transmat_switch = _export(
     "arghmm_assert_transmat_switch", c_bool,
     [c_int, c_int_p, c_int_p, c_int, c_int, c_int, c_int, c_int, c_double_p, c_double_p, c_double])
+argweaver_assert_transmat_internal = _export(
+    "arghmm_assert_transmat_internal", c_bool,
+    [c_int, c_int_p, c_int_p, c_int, c_double_p, c_double_p, c_double])
+argweaver_assert_transmat_switch_internal = _export(
+    "arghmm_assert_transmat_switch_internal", c_bool,
+    [c_void_p, c_int, c_double_p, c_double_p, c_double])
 new_emissions = _export(
     "new_emissions", c_double_p_p,
     [POINTER(intstate), c_int, c_int_p, c_int, c_int_p, c_char_p_p, c_int, c_int, c_double_p,
@@ -115,6 +121,9 @@ new_emissions = _export(
 delete_emissions = _export("delete_emissions", None, [c_double_p_p, c_int])
 argweaver_assert_emit = _export(
     "arghmm_assert_emit", c_bool,
+    [c_void_p, c_int, c_double_p, c_double, c_char_p_p, c_int, c_int])
+argweaver_assert_emit_internal = _export(
+    "arghmm_assert_emit_internal", c_bool,
     [c_void_p, c_int, c_double_p, c_double, c_char_p_p, c_int, c_int])
 forward_alg = _export("forward_alg", None, [c_int, c_int, c_double_p_p, c_double_p_p, c_double_p_p])
 backward_alg = _export("backward_alg", None, [c_int, c_int, c_double_p_p, c_double_p_p, c_double_p_p])
@@ -622,6 +631,26 @@ def assert_transmat_switch(ptree, ages, spr, times, popsizes, rho):
         len(times), _dptr(times), _dptr(popsizes), rho))
 
 
+def assert_transmat_internal(ptree, ages, times, popsizes, rho):
+    """The internal-branch within-block check (draws one irand for the
+    branch lifted above the root)."""
+    ptree = np.ascontiguousarray(ptree, np.int32)
+    ages = np.ascontiguousarray(ages, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return bool(argweaver_assert_transmat_internal(len(ptree), _iptr(ptree), _iptr(ages),
+                                                   len(times), _dptr(times), _dptr(popsizes), rho))
+
+
+def assert_transmat_switch_internal(trees, times, popsizes, rho):
+    """The internal-branch switch check; leaves trees in subtree/maintree
+    form, as the reference does."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return bool(argweaver_assert_transmat_switch_internal(trees, len(times), _dptr(times),
+                                                          _dptr(popsizes), rho))
+
+
 def emissions(ptree, ages, seqs, times, mu, states=None):
     """emit[column, state] of the new sequence seqs[nleaves] (new_emissions)."""
     ptree = np.ascontiguousarray(ptree, np.int32)
@@ -643,6 +672,13 @@ def assert_emit(trees, seqs, times, mu):
     cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
     return bool(argweaver_assert_emit(trees, len(times), _dptr(times), mu, cs.ptr, cs.nseqs,
                                       cs.seqlen))
+
+
+def assert_emit_internal(trees, seqs, times, mu):
+    times = np.ascontiguousarray(times, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return bool(argweaver_assert_emit_internal(trees, len(times), _dptr(times), mu, cs.ptr,
+                                               cs.nseqs, cs.seqlen))
 
 
 def hmm_forward(trans, emit, first):

@@ -15,6 +15,7 @@
 #include "../../include/argweaver_amd.h"
 #include "engine.h"
 #include "serve.h"
+#include "host_util.h"
 #include "local_trees.h"
 #include "thread_path.h"
 #include "internal_prep.h"
@@ -590,18 +591,6 @@ void arghmm_sample_arg_thread_internal(LocalTrees *trees, double *times, int nti
     std::copy(path.begin(), path.end(), thread_path);
 }
 
-// irand (common.h:109-113)
-static int irand_(int max)
-{
-    const int i = int(rand() / float(RAND_MAX) * max);
-    return (i == max) ? max - 1 : i;
-}
-static int irand_(int lo, int hi)
-{
-    const int i = lo + int(rand() / float(RAND_MAX) * (hi - lo));
-    return (i == hi) ? hi - 1 : i;
-}
-
 // resample_arg_all (sample_arg.cpp:111-127): a random branch at a random
 // position, its removal path, then the internal re-threading
 static int resample_arg_all(ArgTrees &tr, const double *times, int ntimes,
@@ -610,8 +599,8 @@ static int resample_arg_all(ArgTrees &tr, const double *times, int ntimes,
 {
     const int maxtime = ntimes + 1;
     std::vector<int> path(tr.blocks.size());
-    const int node = irand_(tr.nnodes);
-    const int pos = irand_(tr.start_coord, tr.end_coord);
+    const int node = irand(tr.nnodes);
+    const int pos = irand(tr.start_coord, tr.end_coord);
     removal_path_at(tr, node, pos, path.data(), prob_path_switch);
     remove_thread_path(tr, path.data(), maxtime, nullptr);
     return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);
@@ -633,7 +622,7 @@ static int resample_arg_leaf(ArgTrees &tr, const double *times, int ntimes,
                              const double *popsizes, double rho, double mu, char **seqs)
 {
     std::vector<int> path(tr.blocks.size());
-    const int node = irand_(tr.num_leaves());
+    const int node = irand(tr.num_leaves());
     removal_leaf_path(tr, node, path.data());
     remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
     return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);

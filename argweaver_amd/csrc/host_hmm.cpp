@@ -20,6 +20,56 @@
 #include "local_trees.h"
 
 namespace aw {
+
+// relative-or-absolute closeness used by the reference's checks (fequal,
+// common.h)
+bool close_enough(double a, double b, double rel, double eabs)
+{
+    if (a == b) return true;
+    const double d = fabs(a - b);
+    if (d <= eabs) return true;
+    return d / fmax(fabs(a), fabs(b)) <= rel;
+}
+
+// likelihood_sites (emit.cpp:328-397) for one state; the callers attach the
+// new branch first (calc_emissions_external_slow, emit.cpp:1138-1168, and
+// calc_emissions_internal_slow, emit.cpp:1171-1214)
+void emission_column_slow(const LTree &t2, const Model &m, const char *const *rows, int seqlen,
+                          const std::vector<char> &invariant, int statei, double **emit)
+{
+    FlatTree ft(t2);
+    const TreeView &t = ft.view;
+    const int N = t.n;
+    const double mintime = m.times[1] * .1;
+    std::vector<double> muts(N, 0.0), nomuts(N, 0.0);
+    double treelen = 0.0;
+    for (int i = 0; i < N; i++) {
+        if (i == t.root) continue;
+        const double d = fmax(m.times[t.age[t.parent[i]]] - m.times[t.age[i]], mintime);
+        muts[i] = prob_branch(d, m.mu, true);
+        nomuts[i] = prob_branch(d, m.mu, false);
+        treelen += d;
+    }
+    const double inv_lk = .25 * exp(-m.mu * fmax(treelen, mintime));
+    std::vector<int> order(N);
+    postorder(t, order.data());
+    std::vector<double> inner(4 * N), outer(4 * N);
+    std::vector<char> bases(N);
+    const int nleaves = (N + 1) / 2;
+    for (int i = 0; i < seqlen; i++) {
+        if (invariant[i]) {
+            emit[i][statei] = (rows[0][i] == 'N') ? 1.0 : inv_lk;
+            continue;
+        }
+        for (int x = 0; x < nleaves; x++) bases[x] = rows[x][i];
+        pruning(t, order.data(), bases.data(), muts.data(), nomuts.data(), inner.data(),
+                outer.data());
+        double lk = 0.0;
+        for (int a = 0; a < 4; a++) lk += .25 * inner[4 * t.root + a];
+        emit[i][statei] = lk;
+    }
+}
+
 namespace {
 
 // rows allocated one by one (new_matrix / delete_matrix, common.h:229-247)
@@ -35,16 +85,6 @@ void delete_rows(double **m, int nrows)
     if (!m) return;
     for (int i = 0; i < nrows; i++) delete[] m[i];
     delete[] m;
-}
-
-// relative-or-absolute closeness used by the reference's checks (fequal,
-// common.h)
-bool close_enough(double a, double b, double rel, double eabs)
-{
-    if (a == b) return true;
-    const double d = fabs(a - b);
-    if (d <= eabs) return true;
-    return d / fmax(fabs(a), fabs(b)) <= rel;
 }
 
 // A local tree built from a parent array (LocalTree(ptree, nnodes, ages),
@@ -226,45 +266,6 @@ void emission_table(const LTree &lt, const Model &m, const char *const *rows, in
             emit[i][j] = emit_variant(in2, &inner[4 * n], &outer[4 * n], n == t.root,
                                       terms[j].mut, terms[j].nomut);
         }
-    }
-}
-
-// likelihood_sites for one state of the tree with the new branch attached
-// (calc_emissions_external_slow, emit.cpp:1138-1168): a full Felsenstein
-// pass over the grown tree per variant column
-void emission_column_slow(const LTree &t2, const Model &m, const char *const *rows, int seqlen,
-                          const std::vector<char> &invariant, int statei, double **emit)
-{
-    FlatTree ft(t2);
-    const TreeView &t = ft.view;
-    const int N = t.n;
-    const double mintime = m.times[1] * .1;
-    std::vector<double> muts(N, 0.0), nomuts(N, 0.0);
-    double treelen = 0.0;
-    for (int i = 0; i < N; i++) {
-        if (i == t.root) continue;
-        const double d = fmax(m.times[t.age[t.parent[i]]] - m.times[t.age[i]], mintime);
-        muts[i] = prob_branch(d, m.mu, true);
-        nomuts[i] = prob_branch(d, m.mu, false);
-        treelen += d;
-    }
-    const double inv_lk = .25 * exp(-m.mu * fmax(treelen, mintime));
-    std::vector<int> order(N);
-    postorder(t, order.data());
-    std::vector<double> inner(4 * N), outer(4 * N);
-    std::vector<char> bases(N);
-    const int nleaves = (N + 1) / 2;
-    for (int i = 0; i < seqlen; i++) {
-        if (invariant[i]) {
-            emit[i][statei] = (rows[0][i] == 'N') ? 1.0 : inv_lk;
-            continue;
-        }
-        for (int x = 0; x < nleaves; x++) bases[x] = rows[x][i];
-        pruning(t, order.data(), bases.data(), muts.data(), nomuts.data(), inner.data(),
-                outer.data());
-        double lk = 0.0;
-        for (int a = 0; a < 4; a++) lk += .25 * inner[4 * t.root + a];
-        emit[i][statei] = lk;
     }
 }
 

@@ -3,6 +3,7 @@
 // probability of one SPR.
 #pragma once
 
+#include <cstdlib>
 #include <vector>
 
 #include "aw_math.h"
@@ -35,5 +36,25 @@ struct FlatTree {
 // leads out of tree t (recombination point, then re-coalescence), given the
 // tree's length
 double spr_logprob(const TreeView &t, const Model &m, const Spr &spr, double treelen);
+
+// irand (common.h:109-113): uniform integer in [lo, hi) from the
+// process-global glibc stream, drawn through a float as the reference does
+inline int irand(int lo, int hi)
+{
+    const int i = lo + int(rand() / float(RAND_MAX) * (hi - lo));
+    return (i == hi) ? hi - 1 : i;
+}
+inline int irand(int max) { return irand(0, max); }
+
+// relative-or-absolute closeness used by the reference's checks (fequal,
+// common.h)
+bool close_enough(double a, double b, double rel, double eabs);
+
+// likelihood_sites for state statei of a complete tree t2 (emit.cpp:328-397):
+// the invariant-site likelihood for columns marked invariant (1 for a masked
+// column), a full Felsenstein pass over the tree for the others; rows[i] is
+// leaf i's sequence
+void emission_column_slow(const LTree &t2, const Model &m, const char *const *rows, int seqlen,
+                          const std::vector<char> &invariant, int statei, double **emit);
 
 }  // namespace aw

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "../../include/argweaver_amd.h"
 #include "engine.h"
 #include "host_util.h"
 #include "internal_prep.h"
@@ -25,6 +26,18 @@
 
 namespace aw {
 namespace {
+
+struct SwitchOut {
+    std::vector<int> determ;
+    std::vector<double> determprob, recombrow, recoalrow;
+    int recombsrc = -1, recoalsrc = -1;
+    double get(int i, int j) const
+    {
+        if (i == recoalsrc) return recoalrow[j];
+        if (i == recombsrc) return recombrow[j];
+        return determ[i] == j ? determprob[i] : 0.0;
+    }
+};
 
 struct BlockStates {
     std::vector<int> node, time;
@@ -390,6 +403,168 @@ void partials(const LTree &t, const EmitTree &et, const char *bases, std::vector
     }
 }
 
+// The switch matrix from tree `last` to tree `t` in compressed form
+// (calc_transition_probs_switch with internal, trans.cpp:538-739): rows of
+// last's states (determ/determprob), the two dense rows over t's states.  A
+// tree without states counts as one placeholder state.
+int switch_internal(const LTree &last, const LTree &t, const BlockStates &s1,
+                    const BlockStates &s, const Spr &spr, const int *mapping, const Model &m,
+                    SwitchOut &o)
+{
+    const int T = m.T, S1 = s1.nstates(), S = s.nstates();
+    o.determ.assign(std::max(S1, 1), -1);
+    o.determprob.assign(std::max(S1, 1), 0.0);
+    o.recombrow.assign(std::max(S, 1), 0.0);
+    o.recoalrow.assign(std::max(S, 1), 0.0);
+    o.recombsrc = o.recoalsrc = -1;
+    if (S1 == 0) {
+        if (S == 0) {
+            o.determ[0] = 0;
+        } else {
+            const int k = s.lookup(main_root(t), spr.ct);
+            if (k < 0) return 1;
+            o.determ[0] = k;
+        }
+        o.determprob[0] = 1.0;
+        return 0;
+    }
+    Lineages PL;
+    lineages_internal(last, T, PL);
+    const double last_len = treelen_internal(last, m.times);
+    const int rpa0 = last.nodes[last.nodes[spr.rn].parent].age;
+    if (S == 0) {
+        for (int i = 0; i < S1; i++) {
+            const int n1 = s1.node[i], t1 = s1.time[i];
+            const int rpa = (n1 == spr.rn && t1 > spr.rt) ? t1 : rpa0;
+            o.determ[i] = 0;
+            o.determprob[i] = recomb_recoal_internal(last, m, PL, spr, n1, t1, rpa, last_len);
+        }
+        return 0;
+    }
+    // calc_recoal_sums (trans.cpp:417-440)
+    double sums = 0.0, sums2[2 * MAXT + 1] = {0.0};
+    for (int x = 2 * spr.rt; x < 2 * spr.ct - 1; x++) {
+        const int nbm = PL.nb[x / 2] - int(x / 2 < rpa0);
+        sums += m.csteps[x] * nbm / (2.0 * m.pop[x / 2]);
+    }
+    double acc = 0.0;
+    sums2[2 * spr.rt] = 0.0;
+    for (int x = 2 * spr.rt; x < 2 * spr.ct - 1; x++) {
+        acc += m.csteps[x] / (2.0 * m.pop[x / 2]);
+        sums2[x + 1] = acc;
+    }
+    double recoals[MAXT];
+    for (int a = 0; a < T; a++) recoals[a] = recoal_internal(last, m, PL, spr, a, rpa0);
+    for (int i = 0; i < S1; i++) {
+        const int n1 = s1.node[i], t1 = s1.time[i];
+        const int tgt = determ_internal(last, t, s, spr, mapping, n1, t1);
+        o.determ[i] = tgt;
+        if (tgt < 0) continue;
+        if (n1 == spr.rn && t1 > spr.rt) {
+            o.determprob[i] = recomb_recoal_internal(last, m, PL, spr, n1, t1, t1, last_len);
+        } else {
+            const int idx = std::max(std::min(2 * spr.ct - 1, 2 * t1), 2 * spr.rt);
+            o.determprob[i] = recomb_internal(last, m, PL, spr, n1, t1, last_len) *
+                              exp(-sums - sums2[idx]) * recoals[t1];
+        }
+    }
+    int rs = -1, cs = -1;
+    for (int i = 0; i < S1; i++) {
+        if (s1.node[i] == spr.rn && s1.time[i] == spr.rt) rs = i;
+        else if (s1.node[i] == spr.cn && s1.time[i] == spr.ct) cs = i;
+    }
+    o.recombsrc = rs;
+    o.recoalsrc = cs;
+    if (rs != -1) {
+        // get_recomb_transition_switch (trans.cpp:278-313)
+        const int parent = last.nodes[spr.rn].parent;
+        const int time2 = last.nodes[parent].age;
+        const int *c = last.nodes[parent].child;
+        const int other = (c[1] == spr.rn ? c[0] : c[1]);
+        const int node2 = (other == spr.cn) ? t.nodes[mapping[other]].parent : mapping[other];
+        const int stay = s.lookup(mapping[spr.rn], spr.rt);
+        const int esc = s.lookup(node2, time2);
+        if (stay != -1)
+            o.recombrow[stay] =
+                recomb_recoal_internal(last, m, PL, spr, s1.node[rs], s1.time[rs], rpa0, last_len);
+        if (esc != -1)
+            o.recombrow[esc] = recomb_recoal_internal(last, m, PL, spr, s1.node[rs], s1.time[rs],
+                                                      s1.time[rs], last_len);
+    }
+    if (cs != -1) {
+        const int node1 = s1.node[cs], time1 = s1.time[cs];
+        const int lp = last.nodes[spr.rn].parent;
+        int node3;
+        if (lp == node1) {
+            const int *c = last.nodes[lp].child;
+            node3 = mapping[c[1] == spr.rn ? c[0] : c[1]];
+        } else {
+            node3 = mapping[node1];
+        }
+        const int parent = t.nodes[mapping[spr.rn]].parent;
+        for (int k = 0; k < S; k++) {
+            const int n2 = s.node[k], t2 = s.time[k];
+            if (!((n2 == mapping[spr.rn] && t2 >= spr.rt) || (n2 == node3 && t2 == time1) ||
+                  (n2 == parent && t2 == time1)))
+                continue;
+            Spr s2 = spr;
+            s2.ct = t2;
+            o.recoalrow[k] = recomb_recoal_internal(last, m, PL, s2, node1, time1, rpa0, last_len);
+        }
+    }
+    return 0;
+}
+
+// The emissions of one tree in subtree/maintree form for its states
+// (calc_emissions with internal, emit.cpp:650-869): per-state branch terms
+// and invariant-column likelihoods, then one variant column at a time.
+struct EmitInternal {
+    const LTree &t;
+    const BlockStates &s;
+    EmitTree et;
+    int sub, mr;
+    std::vector<double> inv, mut, nomut;   // [S], [3S], [3S]
+    std::vector<double> inner, outer;
+    EmitInternal(const LTree &t_, const Model &m, const BlockStates &s_)
+        : t(t_), s(s_), et(t_, m), sub(sub_root(t_)), mr(main_root(t_))
+    {
+        const int S = s.nstates(), T = m.T;
+        const double mintime = m.times[1] * .1;
+        const double time1 = m.times[t.nodes[sub].age];
+        inv.resize(S);
+        mut.resize(3 * S);
+        nomut.resize(3 * S);
+        for (int k = 0; k < S; k++) {
+            const int n = s.node[k], par = t.nodes[n].parent;
+            const double time2 = m.times[t.nodes[n].age];
+            const double pt = (par != -1) ? m.times[std::min(t.nodes[par].age, T - 1)] : 0.0;
+            const double ctm = m.times[s.time[k]];
+            const double dist[3] = {fmax(ctm - time1, mintime), fmax(ctm - time2, mintime),
+                                    fmax(pt - ctm, mintime)};
+            for (int x = 0; x < 3; x++) {
+                mut[3 * k + x] = prob_branch(dist[x], m.mu, true);
+                nomut[3 * k + x] = prob_branch(dist[x], m.mu, false);
+            }
+            double len = et.mainlen + et.sublen + fmax(ctm - time1, mintime);
+            if (n == mr) len += fmax(ctm - m.times[t.nodes[mr].age], mintime);
+            inv[k] = .25 * exp(-m.mu * fmax(len, mintime));
+        }
+        inner.assign(4 * t.nnodes, 0.0);
+        outer.assign(4 * t.nnodes, 0.0);
+    }
+    // the states' emissions of a variant column (bases[i] of leaf i)
+    void variant(const char *bases, double *row)
+    {
+        partials(t, et, bases, inner, outer);
+        const double *in2 = &inner[4 * sub];
+        for (int k = 0; k < s.nstates(); k++) {
+            const int n = s.node[k];
+            row[k] = emit_variant(in2, &inner[4 * n], &outer[4 * n], n == mr, &mut[3 * k],
+                                  &nomut[3 * k]);
+        }
+    }
+};
+
 }  // namespace
 
 int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage, HostPrep &hp)
@@ -397,7 +572,6 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
     const int T = m.T, T1 = T - 1, B = (int)tr.blocks.size(), N = tr.nnodes;
     const int nleaves = tr.num_leaves();
     const int L = tr.length();
-    const double mintime = m.times[1] * .1;
     std::vector<BlockStates> bs(B);
     hp.S.assign(B, 1);
     hp.Sreal.assign(B, 0);
@@ -435,7 +609,6 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
     std::vector<const char *> rows(nleaves);
     for (int i = 0; i < nleaves; i++) rows[i] = seqs[tr.seqids[i]] + tr.start_coord;
 
-    std::vector<double> inner(4 * N), outer(4 * N);
     std::vector<char> bases(N, 'N');
     int end = 0;
     for (int b = 0; b < B; b++) {
@@ -468,10 +641,7 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
             for (int c = 0, x = 0; c < T1; c++)
                 for (int a = c; a < T1; a++)
                     for (int bb = c; bb < T1; bb++, x++) tm2[x] = get_time_min(tv, a, bb, c, mina, true);
-            // emission terms of the states (emit.cpp:778-819)
-            EmitTree et(t, m);
-            const int sub = sub_root(t), mr = main_root(t);
-            const double time1 = m.times[t.nodes[sub].age];
+            EmitInternal em(t, m, s);
             for (int k = 0; k < S; k++) {
                 const int n = s.node[k], a = s.time[k], c = t.nodes[n].age;
                 const int p = lane_pos(k, R);
@@ -485,15 +655,13 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
                 const double term1 = tv.D[a] * tv.E[a];
                 pl[CJ * RW + p] = (term1 * ((2 * (low + tv.G3[a])) - ct) + tv.norecombs[a]) -
                                   term1 * (low + tv.G3[a]);
-                const double coal_time = m.times[a];
-                double len = et.mainlen + et.sublen + fmax(coal_time - time1, mintime);
-                if (n == mr) len += fmax(coal_time - m.times[t.nodes[mr].age], mintime);
-                pl[CE * RW + p] = .25 * exp(-m.mu * fmax(len, mintime));
+                pl[CE * RW + p] = em.inv[k];
                 info[p] = a | (k == s.first[n] ? INFO_FIRST : 0) | (a == s.hi[n] ? INFO_LAST : 0) |
                           INFO_VALID | (c << 11) | (n << 16);
                 if (b == 0) hp.prior[p] = prior_internal(a, Lc, m, minage);
             }
             // columns: masked / invariant / variant over the leaves' rows
+            std::vector<double> row(S);
             for (int col = start; col < end; col++) {
                 const char c0 = rows[0][col];
                 bool inv = true;
@@ -510,23 +678,8 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
                 const size_t off = hp.vemit.size();
                 hp.vemit.resize(off + RW, 0.0);
                 for (int x = 0; x < nleaves; x++) bases[x] = rows[x][col];
-                partials(t, et, bases.data(), inner, outer);
-                const double *in2 = &inner[4 * sub];
-                for (int k = 0; k < S; k++) {
-                    const int n = s.node[k], par = t.nodes[n].parent;
-                    const double time2 = m.times[t.nodes[n].age];
-                    const double pt = (par != -1) ? m.times[std::min(t.nodes[par].age, T - 1)] : 0.0;
-                    const double ctm = m.times[s.time[k]];
-                    const double dist[3] = {fmax(ctm - time1, mintime), fmax(ctm - time2, mintime),
-                                            fmax(pt - ctm, mintime)};
-                    double mu3[3], nm3[3];
-                    for (int x = 0; x < 3; x++) {
-                        mu3[x] = prob_branch(dist[x], m.mu, true);
-                        nm3[x] = prob_branch(dist[x], m.mu, false);
-                    }
-                    hp.vemit[off + lane_pos(k, R)] =
-                        emit_variant(in2, &inner[4 * n], &outer[4 * n], n == mr, mu3, nm3);
-                }
+                em.variant(bases.data(), row.data());
+                for (int k = 0; k < S; k++) hp.vemit[off + lane_pos(k, R)] = row[k];
             }
         }
         if (S == 0)
@@ -534,110 +687,22 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
 
         if (b == 0) continue;
         // the switch from block b-1 (calc_transition_probs_switch, internal)
-        const LTree &last = tr.blocks[b - 1].tree;
-        const BlockStates &s1 = bs[b - 1];
-        const int S1 = s1.nstates();
-        const Spr &spr = BL.spr;
-        const int *mapping = BL.mapping.data();
+        SwitchOut sw;
+        if (switch_internal(tr.blocks[b - 1].tree, t, bs[b - 1], s, BL.spr, BL.mapping.data(), m,
+                            sw))
+            return 2;
         double *cdp = &pl[CDP * RW], *crr = &pl[CRR * RW], *ccr = &pl[CCR * RW];
         int *det = &hp.det[(size_t)b * RW];
-        if (S1 == 0) {
-            if (S == 0) {
-                det[lane_pos(0, R)] = 0;
-                cdp[lane_pos(0, R)] = 1.0;
-            } else {
-                const int k = s.lookup(main_root(t), spr.ct);
-                if (k < 0) return 2;
-                det[lane_pos(0, R)] = k;
-                cdp[lane_pos(0, R)] = 1.0;
-            }
-            continue;
+        for (size_t i = 0; i < sw.determ.size(); i++) {
+            det[lane_pos((int)i, R)] = sw.determ[i];
+            cdp[lane_pos((int)i, R)] = sw.determprob[i];
         }
-        Lineages PL;
-        lineages_internal(last, T, PL);
-        const double last_len = treelen_internal(last, m.times);
-        const int rpa0 = last.nodes[last.nodes[spr.rn].parent].age;
-        if (S == 0) {
-            for (int i = 0; i < S1; i++) {
-                const int n1 = s1.node[i], t1 = s1.time[i];
-                const int rpa = (n1 == spr.rn && t1 > spr.rt) ? t1 : rpa0;
-                det[lane_pos(i, R)] = 0;
-                cdp[lane_pos(i, R)] = recomb_recoal_internal(last, m, PL, spr, n1, t1, rpa, last_len);
-            }
-            continue;
+        for (int k = 0; k < S; k++) {
+            crr[lane_pos(k, R)] = sw.recombrow[k];
+            ccr[lane_pos(k, R)] = sw.recoalrow[k];
         }
-        // calc_recoal_sums (trans.cpp:417-440)
-        double sums = 0.0, sums2[2 * MAXT + 1] = {0.0};
-        for (int x = 2 * spr.rt; x < 2 * spr.ct - 1; x++) {
-            const int nbm = PL.nb[x / 2] - int(x / 2 < rpa0);
-            sums += m.csteps[x] * nbm / (2.0 * m.pop[x / 2]);
-        }
-        double acc = 0.0;
-        sums2[2 * spr.rt] = 0.0;
-        for (int x = 2 * spr.rt; x < 2 * spr.ct - 1; x++) {
-            acc += m.csteps[x] / (2.0 * m.pop[x / 2]);
-            sums2[x + 1] = acc;
-        }
-        double recoals[MAXT];
-        for (int a = 0; a < T; a++) recoals[a] = recoal_internal(last, m, PL, spr, a, rpa0);
-        for (int i = 0; i < S1; i++) {
-            const int n1 = s1.node[i], t1 = s1.time[i];
-            const int tgt = determ_internal(last, t, s, spr, mapping, n1, t1);
-            det[lane_pos(i, R)] = tgt;
-            if (tgt < 0) continue;
-            if (n1 == spr.rn && t1 > spr.rt) {
-                cdp[lane_pos(i, R)] = recomb_recoal_internal(last, m, PL, spr, n1, t1, t1, last_len);
-            } else {
-                const int idx = std::max(std::min(2 * spr.ct - 1, 2 * t1), 2 * spr.rt);
-                cdp[lane_pos(i, R)] = recomb_internal(last, m, PL, spr, n1, t1, last_len) *
-                                      exp(-sums - sums2[idx]) * recoals[t1];
-            }
-        }
-        int rs = -1, cs = -1;
-        for (int i = 0; i < S1; i++) {
-            if (s1.node[i] == spr.rn && s1.time[i] == spr.rt) rs = i;
-            else if (s1.node[i] == spr.cn && s1.time[i] == spr.ct) cs = i;
-        }
-        hp.srcs[2 * b] = rs;
-        hp.srcs[2 * b + 1] = cs;
-        if (rs != -1) {
-            // get_recomb_transition_switch (trans.cpp:278-313)
-            const int parent = last.nodes[spr.rn].parent;
-            const int time2 = last.nodes[parent].age;
-            const int *c = last.nodes[parent].child;
-            const int other = (c[1] == spr.rn ? c[0] : c[1]);
-            const int node2 = (other == spr.cn) ? t.nodes[mapping[other]].parent : mapping[other];
-            const int stay = s.lookup(mapping[spr.rn], spr.rt);
-            const int esc = s.lookup(node2, time2);
-            if (stay != -1)
-                crr[lane_pos(stay, R)] = recomb_recoal_internal(last, m, PL, spr, s1.node[rs],
-                                                                s1.time[rs], rpa0, last_len);
-            if (esc != -1)
-                crr[lane_pos(esc, R)] = recomb_recoal_internal(last, m, PL, spr, s1.node[rs],
-                                                               s1.time[rs], s1.time[rs], last_len);
-        }
-        if (cs != -1) {
-            const int node1 = s1.node[cs], time1 = s1.time[cs];
-            const int lp = last.nodes[spr.rn].parent;
-            int node3;
-            if (lp == node1) {
-                const int *c = last.nodes[lp].child;
-                node3 = mapping[c[1] == spr.rn ? c[0] : c[1]];
-            } else {
-                node3 = mapping[node1];
-            }
-            const int parent = t.nodes[mapping[spr.rn]].parent;
-            for (int k = 0; k < S; k++) {
-                const int n2 = s.node[k], t2 = s.time[k];
-                if (!((n2 == mapping[spr.rn] && t2 >= spr.rt) || (n2 == node3 && t2 == time1) ||
-                      (n2 == parent && t2 == time1)))
-                    continue;
-                Spr s2 = spr;
-                s2.ct = t2;
-                ccr[lane_pos(k, R)] =
-                    recomb_recoal_internal(last, m, PL, s2, node1, time1, rpa0, last_len);
-            }
-        }
+        hp.srcs[2 * b] = sw.recombsrc;
+        hp.srcs[2 * b + 1] = sw.recoalsrc;
     }
     return 0;
 }
@@ -746,3 +811,224 @@ void sample_recombinations_internal(const ArgTrees &tr, const Model &m, int mina
 }
 
 }  // namespace aw
+
+// ---------------------------------------------------------------------------
+// The reference's brute-force checks of the internal-branch matrices: every
+// entry against the probability it stands for, computed on trees with the
+// subtree attached at the states (the test suite's assert_* exports).
+
+namespace aw {
+namespace {
+
+Model check_model(const double *times, int ntimes, const double *popsizes, double rho, double mu)
+{
+    std::vector<double> pops(ntimes, 1.0);
+    if (popsizes)
+        for (int i = 0; i < ntimes; i++) pops[i] = popsizes[i];
+    Model m;
+    model_init(m, ntimes, times, pops.data(), rho, mu);
+    return m;
+}
+
+bool check_grid(const char *who, int ntimes)
+{
+    if (ntimes >= 3 && ntimes <= MAXT) return true;
+    fprintf(stderr, "argweaver_amd: %s: ntimes %d outside 3..%d\n", who, ntimes, MAXT);
+    return false;
+}
+
+// calc_recomb_prob (trans.cpp:849-853)
+double recomb_prob(const LTree &t, const Model &m)
+{
+    FlatTree ft(t);
+    return 1.0 - exp(-fmax(m.rho * tree_length(ft.view, m.times), m.rho));
+}
+
+// calc_spr_prob (total_prob.cpp:210-257) on tree t with its own length
+double spr_prob_on(const LTree &t, const Model &m, const Spr &spr)
+{
+    FlatTree ft(t);
+    return spr_logprob(ft.view, m, spr, tree_length(ft.view, m.times));
+}
+
+}  // namespace
+}  // namespace aw
+
+using namespace aw;
+
+extern "C" {
+
+// assert_transmat_internal (trans.cpp:1035-1121) behind
+// arghmm_assert_transmat_internal (trans.cpp:1314-1343): a random branch
+// lifted above the root, then every entry of the within-block matrix
+// against the sum over the SPRs that lead from state i to state j
+bool arghmm_assert_transmat_internal(int nnodes, int *ptree, int *ages, int ntimes,
+                                     double *times, double *popsizes, double rho)
+{
+    if (!check_grid("arghmm_assert_transmat_internal", ntimes)) return false;
+    const Model m = check_model(times, ntimes, popsizes, rho, 0.0);
+    LTree tree;
+    tree_from_parents(tree, ptree, ages, nnodes, nnodes + 2);
+    const int node = irand(nnodes);
+    apply_spr(tree, Spr{node, tree.nodes[node].age, tree.root, ntimes + 1});
+    int *c = tree.nodes[tree.root].child;
+    if (c[0] != node) std::swap(c[0], c[1]);
+
+    BlockStates s;
+    make_states(tree, ntimes, 0, s);
+    const int S = s.nstates();
+    if (S == 0) return true;
+    Lineages Lc{};
+    TransVec tv{};
+    lineages_internal(tree, ntimes, Lc);
+    trans_internal(tree, m, Lc, tv);
+    const int sub = sub_root(tree), suba = tree.nodes[sub].age;
+    LTree t2 = tree;
+    for (int i = 0; i < S; i++)
+        for (int j = 0; j < S; j++) {
+            const int n1 = s.node[i], a1 = s.time[i], n2 = s.node[j], a2 = s.time[j];
+            const int sister_age = t2.nodes[n1].age;
+            apply_spr(t2, Spr{sub, suba, n1, a1});
+            const double rp = recomb_prob(t2, m);
+            double p = -INFINITY;
+            // the recombination on the new branch, re-coalescing at state j
+            Spr spr{sub, 0, n2, a2};
+            for (int r = t2.nodes[sub].age; r <= std::min(a1, a2); r++) {
+                spr.rt = r;
+                p = logadd(p, spr_prob_on(t2, m, spr));
+            }
+            if (n1 == n2) {
+                // the recombination on the sister branch, re-coalescing on the
+                // new branch or on the node above it
+                spr.rn = n1;
+                spr.cn = (a2 < a1) ? sub : t2.nodes[sub].parent;
+                spr.ct = a2;
+                for (int r = sister_age; r <= std::min(a1, a2); r++) {
+                    spr.rt = r;
+                    p = logadd(p, spr_prob_on(t2, m, spr));
+                }
+            }
+            p += log(rp);
+            if (i == j) p = logadd(p, log(1.0 - rp));
+            const double p2 = log(get_time_min(tv, a1, a2, tree.nodes[n2].age, suba, n1 == n2));
+            if (!close_enough(p, p2, 1e-4, 1e-9)) return false;
+            apply_spr(t2, Spr{sub, t2.nodes[sub].age, t2.root, ntimes + 1});
+        }
+    return true;
+}
+
+// assert_transmat_switch_internal (trans.cpp:1124-1186) behind
+// arghmm_assert_transmat_switch_internal (trans.cpp:1346-1379): a random
+// branch removed along a path from the first tree (the caller's ARG is left
+// in subtree/maintree form, as the reference leaves it), then every nonzero
+// entry of every switch matrix against the probability of the SPR it
+// implies once both trees carry the subtree
+bool arghmm_assert_transmat_switch_internal(LocalTrees *trees, int ntimes, double *times,
+                                            double *popsizes, double rho)
+{
+    if (!trees || !check_grid("arghmm_assert_transmat_switch_internal", ntimes)) return false;
+    const Model m = check_model(times, ntimes, popsizes, rho, 0.0);
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    std::vector<int> path(tr.blocks.size());
+    const int node = irand(tr.nnodes);
+    removal_path_from_start(tr, node, path.data());
+    remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
+
+    const int B = (int)tr.blocks.size();
+    std::vector<BlockStates> bs(B);
+    for (int b = 0; b < B; b++) make_states(tr.blocks[b].tree, ntimes, 0, bs[b]);
+    for (int b = 1; b < B; b++) {
+        const LBlock &BL = tr.blocks[b];
+        const LTree &last = tr.blocks[b - 1].tree, &t = BL.tree;
+        const BlockStates &s1 = bs[b - 1], &s2 = bs[b];
+        SwitchOut sw;
+        if (switch_internal(last, t, s1, s2, BL.spr, BL.mapping.data(), m, sw)) return false;
+        const int last_sub = sub_root(last), sub = sub_root(t);
+        for (int i = 0; i < s1.nstates(); i++)
+            for (int j = 0; j < s2.nstates(); j++) {
+                const double p2 = log(sw.get(i, j));
+                if (p2 == -INFINITY) continue;
+                LTree last2 = last, t2 = t;
+                Spr spr = BL.spr;
+                std::vector<int> mapping(BL.mapping);
+                apply_spr(last2, Spr{last_sub, last2.nodes[last_sub].age, s1.node[i], s1.time[i]});
+                apply_spr(t2, Spr{sub, t2.nodes[sub].age, s2.node[j], s2.time[j]});
+                add_spr_branch_internal(t2, last2, s2.node[j], s2.time[j], s1.node[i], spr, mapping,
+                                        sub, last_sub);
+                const double p = log(recomb_prob(last2, m)) + spr_prob_on(last2, m, spr);
+                if (!close_enough(p, p2, 1e-4, 1e-9)) return false;
+            }
+    }
+    return true;
+}
+
+// assert_emissions_internal (emit.cpp:1253-1280) behind
+// arghmm_assert_emit_internal (emit.cpp:1340-1380): a uniformly drawn
+// removal path applied to a copy of the ARG, then per block the emissions
+// of the internal states against a full likelihood of the tree with the
+// subtree attached at each state (calc_emissions_internal_slow,
+// emit.cpp:1171-1214); seqs[i] is leaf i's sequence
+bool arghmm_assert_emit_internal(LocalTrees *trees, int ntimes, double *times, double mu,
+                                 char **seqs, int nseqs, int seqlen)
+{
+    (void)seqlen;
+    if (!trees || !check_grid("arghmm_assert_emit_internal", ntimes)) return false;
+    const Model m = check_model(times, ntimes, nullptr, 0.0, mu);
+    ArgTrees tr = *reinterpret_cast<ArgTrees *>(trees);
+    std::vector<int> path(tr.blocks.size());
+    removal_path_uniform(tr, path.data());
+    remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
+
+    std::vector<const char *> rows(nseqs);
+    std::vector<char> bases(tr.nnodes, 'N');
+    int end = tr.start_coord;
+    for (const LBlock &BL : tr.blocks) {
+        const int start = end, len = BL.blocklen;
+        end = start + len;
+        const LTree &t = BL.tree;
+        BlockStates s;
+        make_states(t, ntimes, 0, s);
+        const int S = s.nstates();
+        if (S == 0) continue;
+        for (int j = 0; j < nseqs; j++) rows[j] = seqs[j] + start;
+        std::vector<char> inv(len);
+        for (int i = 0; i < len; i++) {
+            bool same = true;
+            for (int x = 1; x < nseqs; x++)
+                if (rows[x][i] != rows[0][i]) {
+                    same = false;
+                    break;
+                }
+            inv[i] = same;
+        }
+        // the fast table (calc_emissions_internal)
+        std::vector<double> fast((size_t)len * S);
+        EmitInternal em(t, m, s);
+        const int nleaves = t.num_leaves();
+        for (int i = 0; i < len; i++) {
+            double *row = &fast[(size_t)i * S];
+            if (inv[i]) {
+                for (int k = 0; k < S; k++) row[k] = (rows[0][i] == 'N') ? 1.0 : em.inv[k];
+                continue;
+            }
+            for (int x = 0; x < nleaves; x++) bases[x] = rows[x][i];
+            em.variant(bases.data(), row);
+        }
+        // the slow table: the subtree attached at each state
+        std::vector<double> slow((size_t)len * S);
+        std::vector<double *> slow_rows(len);
+        for (int i = 0; i < len; i++) slow_rows[i] = &slow[(size_t)i * S];
+        const int sub = sub_root(t), suba = t.nodes[sub].age;
+        LTree t2 = t;
+        for (int k = 0; k < S; k++) {
+            apply_spr(t2, Spr{sub, suba, s.node[k], s.time[k]});
+            emission_column_slow(t2, m, rows.data(), len, inv, k, slow_rows.data());
+            apply_spr(t2, Spr{sub, suba, t2.root, ntimes + 1});
+        }
+        for (size_t x = 0; x < fast.size(); x++)
+            if (!close_enough(fast[x], slow[x], .0001, 1e-12)) return false;
+    }
+    return true;
+}
+
+}  // extern "C"

@@ -440,9 +440,7 @@ void remove_thread_path(ArgTrees &tr, const int *path, int maxtime, int *origina
     remove_null_sprs(tr);
 }
 
-// add_spr_branch for an internal branch (thread.cpp:1204-1309): the SPR and
-// mapping into tree t once both trees carry the re-threaded subtree
-static void add_spr_branch_internal(const LTree &t, const LTree &last, int st_node, int st_time,
+void add_spr_branch_internal(const LTree &t, const LTree &last, int st_node, int st_time,
                                     int last_node, Spr &spr, std::vector<int> &mapping,
                                     int sub, int last_sub)
 {

@@ -32,6 +32,13 @@ void internal_states(const LTree &t, int ntimes, int minage, std::vector<int> &n
 // original_thread is given it receives the removed branch as internal state
 // indices per column
 void remove_thread_path(ArgTrees &tr, const int *path, int maxtime, int *original_thread);
+// add_spr_branch for an internal branch (thread.cpp:1204-1309): rewrite
+// the SPR and extend the mapping into tree t once both trees carry the
+// subtree (st_node: its node in t, -1 when t is complete; last_node: its
+// node in last)
+void add_spr_branch_internal(const LTree &t, const LTree &last, int st_node, int st_time,
+                             int last_node, Spr &spr, std::vector<int> &mapping, int sub,
+                             int last_sub);
 // add the sampled thread (internal state index per column) and its
 // recombinations (node -1 is not used: a recombination on the thread names
 // the subtree root); 0 on success

@@ -167,6 +167,16 @@ bool arghmm_assert_transmat(int nnodes, int *ptree, int *ages, int ntimes, doubl
 bool arghmm_assert_transmat_switch(int nnodes, int *ptree, int *ages, int recomb_node,
                                    int recomb_time, int coal_node, int coal_time, int ntimes,
                                    double *times, double *popsizes, double rho);
+/* src/argweaver/trans.cpp:1314-1343 (assert_transmat_internal, :1035-1121):
+ * a branch drawn with irand lifted above the root, then the within-block
+ * matrix of the internal states checked entry by entry */
+bool arghmm_assert_transmat_internal(int nnodes, int *ptree, int *ages, int ntimes,
+                                     double *times, double *popsizes, double rho);
+/* src/argweaver/trans.cpp:1346-1379 (assert_transmat_switch_internal,
+ * :1124-1186): a branch removed along a path from the first tree (trees is
+ * left in subtree/maintree form), then every switch matrix checked */
+bool arghmm_assert_transmat_switch_internal(LocalTrees *trees, int ntimes, double *times,
+                                            double *popsizes, double rho);
 /* src/argweaver/emit.cpp:1288-1303: emit[seqlen][nstates] for the new
  * sequence seqs[nleaves] against one tree */
 double **new_emissions(intstate *states, int nstates, int *ptree, int nnodes, int *ages_index,
@@ -176,6 +186,11 @@ void delete_emissions(double **emit, int seqlen);
 /* src/argweaver/emit.cpp:1312-1337 (assert_emissions, :1222-1250) */
 bool arghmm_assert_emit(LocalTrees *trees, int ntimes, double *times, double mu, char **seqs,
                         int nseqs, int seqlen);
+/* src/argweaver/emit.cpp:1340-1380 (assert_emissions_internal,
+ * :1253-1280): a uniform removal path applied to a copy of trees, then the
+ * internal-state emissions of every block against the full likelihood */
+bool arghmm_assert_emit_internal(LocalTrees *trees, int ntimes, double *times, double mu,
+                                 char **seqs, int nseqs, int seqlen);
 /* src/argweaver/hmm.cpp:27-44: log-space forward; fw[0] is the caller's */
 void forward_alg(int n, int nstates, double **trans, double **emit, double **fw);
 /* src/argweaver/hmm.cpp:48-64: log-space backward; bw[n-1] is the caller's */

@@ -6,6 +6,10 @@ against the same calls into the reference library compiled from its sources
 - arghmm_assert_transmat / _switch / arghmm_assert_emit (the reference's own
   brute-force checks, trans.cpp:856-1032, emit.cpp:1222-1250) pass, as the
   reference's do on the same trees;
+- arghmm_assert_transmat_internal / _switch_internal / arghmm_assert_emit_internal
+  (the same checks for an internal branch, trans.cpp:1035-1186, 1314-1379,
+  emit.cpp:1171-1280, 1340-1380) pass under the reference's rand() stream,
+  and leave the ARG as the reference leaves it;
 - new_emissions (emit.cpp:1288-1303) with masked columns and 'N' bases;
 - forward_alg / backward_alg / sample_hmm_posterior (hmm.cpp:27-84), the
   posterior sampler under the same rand() stream;
@@ -188,6 +192,76 @@ def test_assert_emit_passes(aw):
     seqs[1, ::17] = ord("N")
     aw.remove_thread(tr, 4)
     assert aw.assert_emit(tr, seqs, times, mu)
+    aw.delete_local_trees(tr)
+
+
+@needs_ref
+@pytest.mark.parametrize("k,ntimes,seed", [(5, 10, 61), (6, 12, 62), (8, 10, 63)])
+def test_assert_transmat_internal_passes_like_reference(aw, k, ntimes, seed):
+    """Internal within-block matrix (minimum age, subtree/maintree lineage
+    counts) against the brute-force SPR sums, for several drawn branches."""
+    tr, _, (P, A, S, bl), (times, pops, rho, mu) = _trees(aw, k, 800, seed, ntimes)
+    fn = _ref_fn("arghmm_assert_transmat_internal", c_bool,
+                 [c_int, c_int_p, c_int_p, c_int, c_double_p, c_double_p, c_double])
+    p, a = np.ascontiguousarray(P[0]), np.ascontiguousarray(A[0])
+    for draw in range(4):
+        O.srand(seed * 10 + draw)
+        ref_ok = fn(len(p), _i(p), _i(a), ntimes, _d(times), _d(pops), rho)
+        O.srand(seed * 10 + draw)
+        assert aw.assert_transmat_internal(p, a, times, pops, rho) == ref_ok
+        assert ref_ok
+    aw.delete_local_trees(tr)
+
+
+@needs_ref
+@pytest.mark.parametrize("k,L,ntimes,seed", [(5, 3000, 10, 71), (7, 5000, 12, 72)])
+def test_assert_transmat_switch_internal_passes_like_reference(aw, k, L, ntimes, seed):
+    """Every internal switch matrix (recombination and re-coalescence rows,
+    deterministic moves, the special cases of a tree left complete) against
+    the SPR probability on the trees carrying the subtree."""
+    tr, _, arrays, (times, pops, rho, mu) = _trees(aw, k, L, seed, ntimes)
+    fn = _ref_fn("arghmm_assert_transmat_switch_internal", c_bool,
+                 [c_void_p, c_int, c_double_p, c_double_p, c_double])
+    for draw in range(3):
+        rt = O.ref_new_trees(*aw.trees_arrays(tr))
+        mine = aw.new_trees(*aw.trees_arrays(tr))
+        O.srand(seed * 10 + draw)
+        ref_ok = fn(c_void_p(rt), ntimes, _d(times), _d(pops), rho)
+        O.srand(seed * 10 + draw)
+        assert aw.assert_transmat_switch_internal(mine, times, pops, rho) == ref_ok
+        assert ref_ok
+        # the ARG is left in subtree/maintree form, node for node as the reference's
+        assert all(np.array_equal(x, y) for x, y in
+                   zip(aw.trees_arrays(mine), O.ref_trees_arrays(rt)))
+        O.ref_delete(rt)
+        aw.delete_local_trees(mine)
+    aw.delete_local_trees(tr)
+
+
+@needs_ref
+@pytest.mark.parametrize("masked", [False, True])
+def test_assert_emit_internal_passes_like_reference(aw, masked):
+    """Internal emissions (subtree partials, maintree root, both tree
+    lengths) against the full pruning pass with the subtree attached at each
+    state, on a uniformly drawn removal path."""
+    tr, seqs, _, (times, pops, rho, mu) = _trees(aw, 6, 600, 81)
+    if masked:
+        seqs = seqs.copy()
+        seqs[2, ::13] = ord("N")
+        seqs[:, ::31] = ord("N")
+    fn = _ref_fn("arghmm_assert_emit_internal", c_bool,
+                 [c_void_p, c_int, c_double_p, c_double, POINTER(c_char_p), c_int, c_int])
+    s = O._Seqs(seqs)
+    for draw in range(3):
+        # a fresh copy per call: the reference's check damages the ARG it is
+        # given (its LocalTrees copy shares the trees), so a second call on
+        # the same one crashes it
+        rt = O.ref_new_trees(*aw.trees_arrays(tr))
+        O.srand(90 + draw)
+        ref_ok = fn(c_void_p(rt), 20, _d(times), mu, s.ptr, seqs.shape[0], seqs.shape[1])
+        O.srand(90 + draw)
+        assert aw.assert_emit_internal(tr, seqs, times, mu) == ref_ok
+        assert ref_ok
     aw.delete_local_trees(tr)
 
 
