@@ -198,6 +198,7 @@ awamd_batch_run_opts = _export(
 awamd_batch_run_ex = _export(
     "awamd_batch_run_ex", c_int,
     [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p])
+awamd_batch_set_traceback = _export("awamd_batch_set_traceback", c_int, [c_void_p, c_int, c_int])
 awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
 awamd_batch_tune_forward = _export(
     "awamd_batch_tune_forward", c_int, [c_void_p, c_int_p, c_int_p, c_int, c_double, c_double_p])
@@ -822,6 +823,13 @@ class Batch(object):
                               int(fw_tile), int(fw_warm), float(fw_tol), _dptr(st)) != 0:
             raise RuntimeError("awamd_batch_run failed")
         return dict(zip(self.STAT_NAMES, st.tolist()))
+
+    def set_traceback(self, warm=256, options=3):
+        """Traceback tiles of later runs: the speculative chain's warm-up
+        columns and the options per tile (1..4); the thread does not depend
+        on them."""
+        if awamd_batch_set_traceback(self.ptr, int(warm), int(options)) != 0:
+            raise ValueError("traceback warm-up / options out of range")
 
     def tune_forward(self, candidates=((0, 0), (60000, 25000), (80000, 25000), (90000, 35000),
                                        (100000, 40000), (110000, 45000), (120000, 50000),

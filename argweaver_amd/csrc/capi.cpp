@@ -759,6 +759,8 @@ int awamd_device_info(char *buf, int n) { return device_info(buf, n); }
 struct AwBatch {
     Batch batch;
     std::vector<HostInstance> staged;
+    int tb_warm = BatchOptions().tile_warm;
+    int tb_opts = BatchOptions().tile_opts;
 };
 
 void *awamd_batch_new(void) { return new AwBatch(); }
@@ -793,6 +795,8 @@ int awamd_batch_run_opts(void *b, int precision, int tile_cols, int traceback, i
     BatchOptions opt;
     opt.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
     opt.tile_cols = tile_cols;
+    opt.tile_warm = ab.tb_warm;
+    opt.tile_opts = ab.tb_opts;
     opt.traceback = traceback != 0;
     opt.fixup = fixup != 0;
     RunStats st;
@@ -811,6 +815,19 @@ int awamd_batch_run_opts(void *b, int precision, int tile_cols, int traceback, i
     return rc;
 }
 
+int awamd_batch_set_traceback(void *b, int warm, int options)
+{
+    AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
+    if (warm < 0 || options < 1 || options > 4) {
+        fprintf(stderr, "argweaver_amd: traceback warm-up %d / options %d out of range\n", warm,
+                options);
+        return -1;
+    }
+    ab.tb_warm = warm;
+    ab.tb_opts = options;
+    return 0;
+}
+
 int awamd_batch_run(void *b, int precision, int tile_cols, int traceback, double *stats)
 {
     return awamd_batch_run_opts(b, precision, tile_cols, traceback, 1, stats);
@@ -823,6 +840,8 @@ int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int
     BatchOptions opt;
     opt.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
     opt.tile_cols = tile_cols;
+    opt.tile_warm = ab.tb_warm;
+    opt.tile_opts = ab.tb_opts;
     opt.traceback = traceback != 0;
     opt.fixup = fixup != 0;
     opt.fw_tile = fw_tile;
@@ -842,8 +861,9 @@ int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int
 
 int awamd_batch_tune(void *b, int precision, const int *cand, int ncand, double *ms_out)
 {
-    return reinterpret_cast<AwBatch *>(b)->batch.tune_tiles(
-        precision == 1 ? Precision::FP32 : Precision::FP64, cand, ncand, ms_out);
+    AwBatch &ab = *reinterpret_cast<AwBatch *>(b);
+    return ab.batch.tune_tiles(precision == 1 ? Precision::FP32 : Precision::FP64, cand, ncand,
+                               ms_out, ab.tb_warm, ab.tb_opts);
 }
 
 int awamd_batch_tune_forward(void *b, const int *tiles, const int *warms, int ncand, double tol,

@@ -102,8 +102,9 @@ struct RunStats {
     double ms_prep = 0, ms_emit = 0, ms_forward = 0, ms_traceback = 0, ms_total = 0;
     long long W = 0;   // column-states
     int nvar = 0;
-    int tb_passes = 0;            // fix-up passes of the tiled traceback
+    int tb_passes = 0;            // re-run passes of the tiled traceback
     long long tb_rerun_cols = 0;  // columns re-sampled by those passes
+    int tb_adopted = 0;           // tiles that took an alternative option
     double ms_fw_main = 0;        // the main forward launch (without re-runs and checks)
     int fw_tiles = 0;             // forward tiles launched
     int fw_reruns = 0;            // forward tiles re-run from their boundary column
@@ -119,16 +120,18 @@ struct HostInstance {
     std::shared_ptr<const HostPrep> prep;
 };
 
-// Schedule of a batch run.  The forward recursion runs one wave per
-// instance over all of its columns (the filter keeps the memory of its start
-// column for ~10^4 columns, so column tiles cannot be started speculatively);
-// the traceback runs one wave per column tile.
+// Schedule of a batch run.  The forward runs one wave per instance over
+// all of its columns, or, in FP32, one wave per forward tile (checked
+// speculative starts); the traceback runs one wave per option of a column
+// tile.
 struct BatchOptions {
     Precision prec = Precision::FP64;
     int tile_cols = 0;       // traceback tile length (0: one tile per instance)
     int tile_warm = 256;     // warm-up columns of a speculative tile start
+    int tile_opts = 3;       // options per traceback tile (1 .. 4): the
+                             // speculative chain and its alternatives
     bool traceback = true;
-    bool fixup = true;       // false: speculative traceback tiles only (diagnostics)
+    bool fixup = true;       // false: speculative option-0 paths only (diagnostics)
     // FP32 forward tiles: long instances are split into tiles (counting the
     // warm-up) of fw_tile columns at a column cost of 0.94 us, scaled by
     // the column cost of the instance's states per lane, that run
@@ -167,8 +170,10 @@ public:
     // the traceback alone for each candidate tile length (0: one tile per
     // instance) on the resident tables; returns the fastest (ms per
     // candidate in ms_out when given), -1 on error.  Leaves the batch with
-    // the path of the last candidate.
-    int tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out = nullptr);
+    // the path of the last candidate.  warm / opts: the tiles' warm-up and
+    // options (BatchOptions::tile_warm, tile_opts).
+    int tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out = nullptr,
+                   int warm = BatchOptions().tile_warm, int opts = BatchOptions().tile_opts);
     // FP32 forward-tile autotuner: times the forward (with its checks and
     // re-runs) for each (tile, warm-up) candidate and returns the index of
     // the fastest (-1 on error)

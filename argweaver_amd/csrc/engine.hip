@@ -24,13 +24,17 @@
 //                  table, the result reaching each state by ds_bpermute; a
 //                  column costs O(S + T^2) and rows are written coalesced.
 //   k_lnl_b        per-instance sum of log normalisers (FP64)
-//   k_tb_all       one wave per column tile, every tile in one launch: FP64
-//                  weights and CDF (DPP wave scan), rows staged through LDS
-//                  in double-buffered batches.  Tiles before the last start
-//                  from a state sampled from a column past their end and a
-//                  warm-up; fix-up passes re-run a tile from the true
-//                  boundary state until the re-sampled path coalesces with
-//                  the stored one (identical from there on).
+//   k_tb_spec      one wave per option of a column tile, every tile in one
+//                  launch: FP64 weights and CDF (DPP wave scan), rows
+//                  staged through LDS in double-buffered batches.  A tile
+//                  before the last runs a speculative chain (a state
+//                  sampled from a column past its end and a warm-up) and
+//                  alternative chains from the neighbouring states of that
+//                  chain's branch; once the true state at the tile's end is
+//                  known the option that started from it is exact
+//   k_tb_rerun     a tile no option fits, re-sampled from its true state
+//                  until it coalesces with an option's path (identical from
+//                  there on)
 //
 // The forward kernel is a template over the table type: double is the
 // drop-in arithmetic (the reference's forward table to ~1e-15), float keeps
@@ -1363,7 +1367,20 @@ struct Chunk {
     int inst, c0, c1;        // columns [c0, c1) of instance inst
     int cw;                  // column where the speculative chain starts (>= c1)
     int first, last;         // first / last tile of its instance
+    long long coff;          // the tile's alternative paths in the option buffer
 };
+
+// Options of a traceback tile.  Option 0 is the speculative chain: a state
+// sampled from column cw alone, a warm-up back to column c1 (not stored),
+// then the tile's columns into the path.  Options 1 .. K-1 are chains of the
+// same tile started at column c1 from the other states of the branch that
+// option 0 holds there (the most probable first, by the forward column):
+// two backward chains on one branch at neighbouring times can run side by
+// side for tens of thousands of columns with the same uniforms, and one of
+// them is then the true chain.  Each option records the state it starts
+// from at c1 and the state it reaches at c0; once the true state at c1 is
+// known, an option that started from it is the tile's exact path.
+constexpr int TB_KMAX = 4;
 
 template <int R>
 struct TbLane {
@@ -1485,46 +1502,65 @@ __device__ __forceinline__ void row_load(AW_GLOBAL const Real *alpha, int col, R
     for (int r = 0; r < R; r++) rv[r] = row[r * 64 + threadIdx.x];
 }
 
-// Sample the tile's columns [lo, hi] given the state knext (info kinfo) at
-// column hi+1 in block kblk.  mode 1 stops at the first column whose sample
-// equals the stored path (the chains have coalesced) and returns it; else
-// returns -1.  The uniform of column col is u[L-1-col].
+// The state of a backward chain: the state k sampled at the column last
+// visited, its info word, and the block of that column.
+struct Chain {
+    int k, info, blk;
+};
+
+// Stored paths a re-run compares its samples with (option 0 in the path,
+// the alternatives in the option buffer), each indexed by column; null for
+// an option not compared.
+struct Cmp {
+    const AW_GLOBAL int *p[TB_KMAX];
+};
+
+// Sample columns hi, hi-1, .., lo given the chain's state at column hi+1,
+// storing the samples of columns <= whi through dst (dst[c] for column c;
+// nothing when dst is null).  The walk stops at the first column whose
+// sample equals a compared stored path o (the chains have coalesced: the
+// rest is that path) and returns it with opt = o; else it returns -1 with
+// the chain at column lo.  The uniform of column col is u[L-1-col].
 //
 // Rows come through LDS in batches of BT columns, double-buffered: the
 // loads of the next batch are issued when a batch starts and written to
 // LDS when it ends, so the column loop itself never waits on HBM (vmcnt
 // orders loads behind stores, so a per-column prefetch would not hide it).
-// The uniforms and, in mode 1, the stored path come 64 columns at a time,
-// one per lane, read with readlane.
+// The uniforms and the stored paths come 64 columns at a time, one per
+// lane, read with readlane.
 template <class Real, int R>
 __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows, int hi, int lo,
-                                        int knext, int kinfo, int kblk, int mode, int whi)
+                                        Chain &chn, AW_GLOBAL int *dst, int whi, const Cmp &cmp,
+                                        int &opt)
 {
     constexpr int RW = 64 * R;
     constexpr int BT = tb_batch<Real, R>();
     const int lane = threadIdx.x, L = d.L;
     AW_GLOBAL const Real *alpha = (AW_GLOBAL const Real *)d.alpha;
-    auto path = gp(d.path);
     const AW_GLOBAL double *ug = gp(d.u);
-    const AW_GLOBAL int *pg = gp(d.path);
+    opt = -1;
     if (hi < lo) return -1;
+    int knext = chn.k, kinfo = chn.info, kblk = chn.blk;
     int bc = find_owner(cp(d.bstart), d.B, hi);
     int bc_start = cp(d.bstart)[bc];
     int S = cp(d.soff)[bc + 1] - cp(d.soff)[bc];
     TbLane<R> tl;
     tb_load_block<Real, R>(d, bc, tl, tab);
     // window of 64 columns w0, w0-1, .., w0-63: lane j holds column w0-j
-    // (its uniform, its stored path in mode 1, and the sampled state, which
-    // is stored once per window: per-column stores would make every later
-    // load wait for them, vmcnt being in order)
+    // (its uniform, the stored paths, and the sampled state, which is
+    // stored once per window: per-column stores would make every later load
+    // wait for them, vmcnt being in order)
     int w0 = hi;
     double uw = ug[min(L - 1 - (w0 - lane), L - 1)];
-    int pw = (mode == 1) ? pg[max(w0 - lane, 0)] : -1;
+    // stored paths are read within the walked columns [lo, hi] only
+    int pw[TB_KMAX];
+#pragma unroll
+    for (int o = 0; o < TB_KMAX; o++) pw[o] = cmp.p[o] ? cmp.p[o][max(w0 - lane, lo)] : -1;
     int pnew = 0;
     // store the sampled states of the window's columns w0 .. clo
     auto flush = [&](int clo) {
         const int c = w0 - lane;
-        if (c >= clo && c <= min(w0, whi)) path[c] = pnew;
+        if (dst && c >= clo && c <= min(w0, whi)) dst[c] = pnew;
     };
     Real nxt[BT][R];
 #pragma unroll
@@ -1544,10 +1580,11 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
                 flush(col + 1);
                 w0 = col;
                 uw = ug[min(L - 1 - (w0 - lane), L - 1)];
-                if (mode == 1) pw = pg[max(w0 - lane, 0)];
+#pragma unroll
+                for (int o = 0; o < TB_KMAX; o++)
+                    if (cmp.p[o]) pw[o] = cmp.p[o][max(w0 - lane, lo)];
             }
             const double uval = readlane_d(uw, w0 - col);
-            const int oldp = (mode == 1) ? __builtin_amdgcn_readlane(pw, w0 - col) : -1;
             const Real *row = rb + (top - col) * RW;
             // every load first, then the weights: per-slot conditions around
             // a load would give each load its own wait
@@ -1583,8 +1620,14 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
             }
             knext = lane_sample<R>(pre, tl.ti, S, uval, kinfo);
             kblk = bc;
-            if (mode == 1 && oldp == knext) {
+            int hit = -1;
+#pragma unroll
+            for (int o = TB_KMAX - 1; o >= 0; o--)
+                if (cmp.p[o] && __builtin_amdgcn_readlane(pw[o], w0 - col) == knext) hit = o;
+            if (hit >= 0) {
                 flush(col + 1);
+                chn = Chain{knext, kinfo, kblk};
+                opt = hit;
                 return col;
             }
             pnew = (lane == w0 - col) ? knext : pnew;
@@ -1604,13 +1647,14 @@ __device__ __forceinline__ int tb_range(const DevInst &d, Real *tab, Real *rows,
         lds_order();
     }
     flush(lo);
+    chn = Chain{knext, kinfo, kblk};
     return -1;
 }
 
 // state sampled from column col alone (the last column, or the speculative
 // start of a tile)
 template <class Real, int R>
-__device__ __forceinline__ int tb_column(const DevInst &d, int col, double uval, int &kinfo, int &blk)
+__device__ __forceinline__ Chain tb_column(const DevInst &d, int col, double uval)
 {
     constexpr int RW = 64 * R;
     AW_GLOBAL const Real *alpha = (AW_GLOBAL const Real *)d.alpha;
@@ -1629,68 +1673,206 @@ __device__ __forceinline__ int tb_column(const DevInst &d, int col, double uval,
         acc += (double)rv[r];   // padding slots hold 0
         pre[r] = acc;
     }
-    blk = b;
-    return lane_sample<R>(pre, ti, S, uval, kinfo);
+    Chain c;
+    c.k = lane_sample<R>(pre, ti, S, uval, c.info);
+    c.blk = b;
+    return c;
 }
 
+// The alternative start of option o >= 1 at column col: among the other
+// states of the branch of state k0 (block b), the one of rank o - 1 by the
+// forward column (ties by state index); -1 when the branch has too few.
+// Lane l looks at state k0 - 31 + l: a branch has fewer than 32 states.
 template <class Real, int R>
-__device__ __forceinline__ int tb_tile(const DevInst d, const Chunk &ch, int mode, const int *snap, void *lds)
+__device__ __forceinline__ int tb_alternative(const DevInst &d, int col, int b, int k0, int o)
+{
+    constexpr int RW = 64 * R;
+    const int lane = threadIdx.x;
+    const int S = cp(d.soff)[b + 1] - cp(d.soff)[b];
+    const AW_GLOBAL int *info = gp(d.linfo) + (size_t)b * RW;
+    AW_GLOBAL const Real *row = (AW_GLOBAL const Real *)d.alpha + (size_t)col * RW;
+    const int node = info_node(info[lane_pos(k0, R)]);
+    const int x = k0 - 31 + lane;
+    const bool ok = x >= 0 && x < S && x != k0 && info_node(info[lane_pos(min(max(x, 0), S - 1), R)]) == node;
+    const double v = ok ? (double)row[lane_pos(min(max(x, 0), S - 1), R)] : -1.0;
+    // rank among the candidates: those with a larger value, or an equal
+    // value and a smaller index
+    int rank = 0;
+    for (int j = 0; j < 64; j++) {
+        const double vj = readlane_d(v, j);
+        rank += (vj > v || (vj == v && j < lane)) ? 1 : 0;
+    }
+    const unsigned long long m = __ballot(ok && rank == o - 1);
+    if (!m) return -1;
+    return __builtin_amdgcn_readlane(x, (int)__ffsll((long long)m) - 1);
+}
+
+// The option buffer column c of option o >= 1 of tile ch (dst[c] / src[c]).
+__device__ __forceinline__ AW_GLOBAL int *opt_path(int *obuf, const Chunk &ch, int o)
+{
+    return (AW_GLOBAL int *)obuf + ch.coff + (long long)(o - 1) * (ch.c1 - ch.c0) - ch.c0;
+}
+
+// One option of a tile (the speculative launch).  ostart / oend receive the
+// option's state at column c1 (-1: no such option) and at column c0.
+template <class Real, int R>
+__device__ __forceinline__ void tb_option(const DevInst d, const Chunk &ch, int o, int K, int *obuf,
+                                          int *ostart, int *oend, void *lds)
 {
     Real *tab = (Real *)lds;
     Real *rows = tab + (d.T - 1) * (d.T - 1) + d.TC;
     constexpr int RW = 64 * R;
     const int L = d.L;
-    int knext, kinfo, kblk, hi;
+    AW_GLOBAL int *path = gp(d.path);
+    const Cmp none{{nullptr, nullptr, nullptr, nullptr}};
+    int dummy;
+    Chain c;
     if (ch.last) {
-        knext = tb_column<Real, R>(d, L - 1, cp(d.u)[0], kinfo, kblk);
-        if (threadIdx.x == 0) gp(d.path)[L - 1] = knext;
-        hi = L - 2;
-    } else if (mode == 0) {
-        // speculative start: a state sampled from column cw alone, then a
-        // warm-up over [c1, cw) whose samples are not stored; by c1 the
-        // chain has usually coalesced with the true one
-        knext = tb_column<Real, R>(d, ch.cw, cp(d.u)[L - 1 - ch.cw], kinfo, kblk);
-        hi = ch.cw - 1;
-    } else {
-        knext = snap[blockIdx.x];
-        kblk = find_owner(cp(d.bstart), d.B, ch.c1);
-        kinfo = gp(d.linfo)[(size_t)kblk * RW + lane_pos(knext, R)];
-        hi = ch.c1 - 1;
+        // the last tile is exact: option 0 only
+        if (o > 0) {
+            if (threadIdx.x == 0) *ostart = *oend = -1;
+            return;
+        }
+        c = tb_column<Real, R>(d, L - 1, cp(d.u)[0]);
+        if (threadIdx.x == 0) path[L - 1] = c.k;
+        tb_range<Real, R>(d, tab, rows, L - 2, ch.c0, c, path, L - 2, none, dummy);
+        if (threadIdx.x == 0) {
+            *ostart = -1;
+            *oend = c.k;
+        }
+        return;
     }
-    return tb_range<Real, R>(d, tab, rows, hi, ch.c0, knext, kinfo, kblk, mode, ch.c1 - 1);
+    // the speculative chain's warm-up, the same in every option
+    c = tb_column<Real, R>(d, ch.cw, cp(d.u)[L - 1 - ch.cw]);
+    tb_range<Real, R>(d, tab, rows, ch.cw - 1, ch.c1, c, nullptr, -1, none, dummy);
+    AW_GLOBAL int *dst = path;
+    if (o > 0) {
+        const int k = tb_alternative<Real, R>(d, ch.c1, c.blk, c.k, o);
+        if (k < 0) {
+            if (threadIdx.x == 0) *ostart = *oend = -1;
+            return;
+        }
+        c.k = k;
+        c.info = gp(d.linfo)[(size_t)c.blk * RW + lane_pos(k, R)];
+        dst = opt_path(obuf, ch, o);
+    }
+    const int start = c.k;
+    tb_range<Real, R>(d, tab, rows, ch.c1 - 1, ch.c0, c, dst, ch.c1 - 1, none, dummy);
+    if (threadIdx.x == 0) {
+        *ostart = start;
+        *oend = c.k;
+    }
+    (void)K;
 }
 
-// every tile of a batch in one launch (see k_fw_all)
+// Re-run of a tile from its true state at column c1 until it coalesces with
+// one of its options' stored paths; the path then follows that option (an
+// alternative's columns are copied in).  res: the column of coalescence (-1
+// for none), the option, and the state at column c0.
+template <class Real, int R>
+__device__ __forceinline__ void tb_rerun(const DevInst d, const Chunk &ch, int k1, int K, int *obuf,
+                                         const int *ostart, int *res, void *lds)
+{
+    Real *tab = (Real *)lds;
+    Real *rows = tab + (d.T - 1) * (d.T - 1) + d.TC;
+    constexpr int RW = 64 * R;
+    AW_GLOBAL int *path = gp(d.path);
+    Cmp cmp{{path, nullptr, nullptr, nullptr}};
+#pragma unroll
+    for (int o = 1; o < TB_KMAX; o++)
+        if (o < K && ostart[o] >= 0) cmp.p[o] = opt_path(obuf, ch, o);
+    Chain c;
+    c.blk = find_owner(cp(d.bstart), d.B, ch.c1);
+    c.k = k1;
+    c.info = gp(d.linfo)[(size_t)c.blk * RW + lane_pos(k1, R)];
+    int hit;
+    const int col = tb_range<Real, R>(d, tab, rows, ch.c1 - 1, ch.c0, c, path, ch.c1 - 1, cmp, hit);
+    int end = c.k;
+    const int o = (col >= 0) ? hit : -1;
+    if (col >= 0) {
+        if (o > 0) {
+            // the rest of the tile is the alternative's
+            const AW_GLOBAL int *src = opt_path(obuf, ch, o);
+            for (int cc = ch.c0 + (int)threadIdx.x; cc <= col; cc += 64) path[cc] = src[cc];
+            end = src[ch.c0];
+        } else {
+            end = path[ch.c0];
+        }
+    }
+    if (threadIdx.x == 0) {
+        res[0] = col;
+        res[1] = o;
+        res[2] = end;
+    }
+}
+
+// Launch of the speculative options: block (j, o) of tile j, option o sits
+// at 8 * (K * (j / 8) + o) + j % 8, so the options of a tile run on one XCD
+// (blocks go round-robin over the eight) and share its L2 for their rows.
 template <class Real>
-__global__ __launch_bounds__(64) void k_tb_all(const DevInst *__restrict__ D,
-                                               const Chunk *__restrict__ chunks, int mode,
-                                               const int *__restrict__ snap,
-                                               int *__restrict__ flags)
+__global__ __launch_bounds__(64) void k_tb_spec(const DevInst *__restrict__ D,
+                                                const Chunk *__restrict__ chunks, int nch, int K,
+                                                int *__restrict__ obuf, int *__restrict__ ostart,
+                                                int *__restrict__ oend)
 {
     extern __shared__ double tb_dyn[];
-    const Chunk ch = chunks[blockIdx.x];
+    const int bx = blockIdx.x;
+    const int j = (bx / (8 * K)) * 8 + bx % 8, o = (bx / 8) % K;
+    if (j >= nch) return;
+    const Chunk ch = chunks[j];
     const DevInst &d = D[ch.inst];
-    int res;
+    int *os = ostart + (size_t)j * K + o, *oe = oend + (size_t)j * K + o;
     switch (d.R) {
-    case 1: res = tb_tile<Real, 1>(d, ch, mode, snap, tb_dyn); break;
-    case 2: res = tb_tile<Real, 2>(d, ch, mode, snap, tb_dyn); break;
-    case 4: res = tb_tile<Real, 4>(d, ch, mode, snap, tb_dyn); break;
-    case 6: res = tb_tile<Real, 6>(d, ch, mode, snap, tb_dyn); break;
-    case 8: res = tb_tile<Real, 8>(d, ch, mode, snap, tb_dyn); break;
-    case 12: res = tb_tile<Real, 12>(d, ch, mode, snap, tb_dyn); break;
-    default: res = tb_tile<Real, 16>(d, ch, mode, snap, tb_dyn); break;
+    case 1: tb_option<Real, 1>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    case 2: tb_option<Real, 2>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    case 4: tb_option<Real, 4>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    case 6: tb_option<Real, 6>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    case 8: tb_option<Real, 8>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    case 12: tb_option<Real, 12>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    default: tb_option<Real, 16>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
     }
-    if (mode == 1 && threadIdx.x == 0) flags[blockIdx.x] = res;
 }
 
-// boundary state of every tile before the last: the path at column c1
-__global__ __launch_bounds__(64) void k_tb_snapshot(const DevInst *__restrict__ D,
-                                                    const Chunk *__restrict__ chunks,
-                                                    int *__restrict__ snap)
+// Re-runs: block x re-samples tile idx[x] from state k1[x] at its column c1.
+template <class Real>
+__global__ __launch_bounds__(64) void k_tb_rerun(const DevInst *__restrict__ D,
+                                                 const Chunk *__restrict__ chunks,
+                                                 const int *__restrict__ idx,
+                                                 const int *__restrict__ k1, int K,
+                                                 int *__restrict__ obuf,
+                                                 const int *__restrict__ ostart,
+                                                 int *__restrict__ res)
 {
-    const Chunk ch = chunks[blockIdx.x];
-    if (ch.last || threadIdx.x != 0) return;
-    snap[blockIdx.x] = D[ch.inst].path[ch.c1];
+    extern __shared__ double tb_dyn[];
+    const int j = idx[blockIdx.x];
+    const Chunk ch = chunks[j];
+    const DevInst &d = D[ch.inst];
+    const int *os = ostart + (size_t)j * K;
+    int *rs = res + 3 * blockIdx.x;
+    const int k = k1[blockIdx.x];
+    switch (d.R) {
+    case 1: tb_rerun<Real, 1>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    case 2: tb_rerun<Real, 2>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    case 4: tb_rerun<Real, 4>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    case 6: tb_rerun<Real, 6>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    case 8: tb_rerun<Real, 8>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    case 12: tb_rerun<Real, 12>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    default: tb_rerun<Real, 16>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    }
+}
+
+// Adopted alternatives: block x copies option opt[x] of tile idx[x] into the
+// path over the tile's columns.
+__global__ __launch_bounds__(256) void k_tb_adopt(const DevInst *__restrict__ D,
+                                                  const Chunk *__restrict__ chunks,
+                                                  const int *__restrict__ idx,
+                                                  const int *__restrict__ opt,
+                                                  int *__restrict__ obuf)
+{
+    const Chunk ch = chunks[idx[blockIdx.x]];
+    AW_GLOBAL int *path = gp(D[ch.inst].path);
+    const AW_GLOBAL int *src = opt_path(obuf, ch, opt[blockIdx.x]);
+    for (int c = ch.c0 + (int)threadIdx.x; c < ch.c1; c += 256) path[c] = src[c];
 }
 
 //===========================================================================
@@ -1857,13 +2039,12 @@ int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double
     return 0;
 }
 
-template <class Real>
-int launch_tb(const DevInst *dD, const Chunk *chunks, int cnt, int mode, const int *snap,
-              int *flags, size_t sh, hipStream_t st)
+// launch of a traceback kernel with sh bytes of dynamic LDS
+template <class Kern, class... Args>
+int launch_lds(Kern kern, int grid, size_t sh, hipStream_t st, Args... args)
 {
-    auto kern = k_tb_all<Real>;
     HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, chunks, mode, snap, flags);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), sh, st, args...);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -1887,7 +2068,11 @@ struct Batch::Impl {
     DevInst *dD = nullptr;
     long long *d_coloff = nullptr;
     int *d_blkoff = nullptr, *d_voff = nullptr, *d_nvar = nullptr, *d_list = nullptr;
-    int *d_flags = nullptr, *d_tsnap = nullptr;
+    // traceback options: start / end states [nch * K], alternative paths,
+    // re-run lists and results, adopted alternatives
+    int *d_ostart = nullptr, *d_oend = nullptr, *d_obuf = nullptr;
+    int *d_ridx = nullptr, *d_rk1 = nullptr, *d_rres = nullptr, *d_aidx = nullptr, *d_aopt = nullptr;
+    int tb_opts = 1;
     Chunk *d_chunks = nullptr;
     FwTile *d_fwt = nullptr, *d_fwre = nullptr;
     char *d_bnd = nullptr;
@@ -2082,9 +2267,10 @@ inline double tb_cost_model(int R)
 
 // Traceback tiles of every instance: about tile_cols columns (0: one per
 // instance; scaled by the column cost of the instance's R), the instances
-// with the most states per lane first.  Copies the tile list to the device
-// on the main stream; returns the LDS size of the tile kernel (sht).
-static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t &sht)
+// with the most states per lane first, K options per tile.  Copies the tile
+// list to the device on the main stream; returns the LDS size of the tile
+// kernels (sht).
+static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K, size_t &sht)
 {
     const int n = (int)m.h.size();
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
@@ -2096,6 +2282,7 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
     std::iota(byr.begin(), byr.end(), 0);
     std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return m.h[a].R > m.h[b].R; });
     sht = 0;
+    long long ocols = 0;   // columns of the tiles that have alternatives
     for (int i : byr) {
         const DevInst &d = m.h[i];
         // tile_cols columns at R = 12's column cost; instances with cheaper
@@ -2112,6 +2299,8 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
             k.first = (t == 0);
             k.last = (t == nt - 1);
             k.cw = k.last ? d.L - 1 : std::min(d.L - 1, k.c1 + std::max(warm, 0));
+            k.coff = k.last ? 0 : (long long)(K - 1) * ocols;
+            if (!k.last) ocols += k.c1 - k.c0;
             chs.push_back(k);
         }
         // (a,b) and (c,a,b) tables, then two row batches
@@ -2120,12 +2309,19 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
         sht = std::max(sht, rsz * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
     const int nch = (int)chs.size();
+    m.tb_opts = K;
     for (int pass = 0; pass < 2; pass++) {
         Plan p;
         p.base = pass ? m.tiles.p : nullptr;
-        m.d_flags = p.take<int>(nch);
-        m.d_tsnap = p.take<int>(nch);
         m.d_chunks = p.take<Chunk>(nch);
+        m.d_ostart = p.take<int>((size_t)nch * K);
+        m.d_oend = p.take<int>((size_t)nch * K);
+        m.d_ridx = p.take<int>(nch);
+        m.d_rk1 = p.take<int>(nch);
+        m.d_rres = p.take<int>(3 * (size_t)nch);
+        m.d_aidx = p.take<int>(nch);
+        m.d_aopt = p.take<int>(nch);
+        m.d_obuf = p.take<int>((size_t)std::max(1LL, (long long)(K - 1) * ocols));
         if (pass == 0 && m.tiles.ensure(p.used + 4096)) return 1;
     }
     HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice,
@@ -2133,66 +2329,135 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, size_t
     return 0;
 }
 
-// Fix-up passes: a tile is re-sampled from its true boundary state until it
-// coalesces with the stored path; a tile that does not coalesce passes the
-// re-run on to the tile before it.
-static int fixup_passes(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
+// The speculative launch: every option of every tile.
+static int launch_options(Batch::Impl &m, bool fp32, size_t sht)
 {
-    Streams &c = m.st;
-    hipStream_t s = c.main;
+    const int nch = (int)m.chunks.size(), K = m.tb_opts;
+    const int grid = (nch + 7) / 8 * 8 * K;
+    hipStream_t s = m.st.main;
+    return fp32 ? launch_lds(k_tb_spec<float>, grid, sht, s, (const DevInst *)m.dD,
+                             (const Chunk *)m.d_chunks, nch, K, m.d_obuf, m.d_ostart, m.d_oend)
+                : launch_lds(k_tb_spec<double>, grid, sht, s, (const DevInst *)m.dD,
+                             (const Chunk *)m.d_chunks, nch, K, m.d_obuf, m.d_ostart, m.d_oend);
+}
+
+// Resolution of the tiles, right to left within each instance.  A tile
+// whose true state at c1 is the start of one of its options takes that
+// option's path (option 0 is in place; an alternative is copied in at the
+// end) and passes that option's end state on as the true state of the
+// tile before.  The first tile that no option fits is re-run from its true
+// state, and in the same pass every tile before it whose predicted start
+// (the end of the next tile's option 0, or of the option its own
+// prediction fits) fits none of its options is re-run from that
+// prediction.  A re-run replaces option 0 by the chain from its start (the
+// samples until it coalesces with a stored option, then that option), so a
+// right prediction leaves the tile decided for the next pass.
+static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
+{
+    hipStream_t s = m.st.main;
     const std::vector<Chunk> &chs = m.chunks;
-    const int nch = (int)chs.size();
-    std::vector<int> redo;
+    const int nch = (int)chs.size(), K = m.tb_opts;
+    std::vector<int> os((size_t)nch * K), oe((size_t)nch * K);
+    HIP_OK(hipMemcpyAsync(os.data(), m.d_ostart, sizeof(int) * os.size(), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(oe.data(), m.d_oend, sizeof(int) * oe.size(), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    auto match = [&](int j, int b) {
+        for (int o = 0; o < K; o++)
+            if (os[(size_t)j * K + o] == b) return o;
+        return -1;
+    };
+    // per instance: its first tile, the rightmost undecided tile and the
+    // true state at that tile's column c1
+    struct Walk {
+        int f, r, b;
+    };
+    std::vector<Walk> ws;
     for (int j = 0; j < nch; j++)
-        if (!chs[j].last) redo.push_back(j);
-    std::vector<Chunk> sub;
-    std::vector<int> flags;
-    while (!redo.empty()) {
-        rs.tb_passes++;
-        sub.clear();
-        for (int j : redo) sub.push_back(chs[j]);
-        const int ns = (int)sub.size();
-        HIP_OK(hipMemcpyAsync(m.d_chunks, sub.data(), sizeof(Chunk) * ns, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_tb_snapshot, dim3(ns), dim3(64), 0, s, m.dD, m.d_chunks, m.d_tsnap);
-        if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, ns, 1, m.d_tsnap, m.d_flags, sht, s)
-                 : launch_tb<double>(m.dD, m.d_chunks, ns, 1, m.d_tsnap, m.d_flags, sht, s))
-            return 1;
-        flags.resize(ns);
-        HIP_OK(hipMemcpyAsync(flags.data(), m.d_flags, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        std::vector<int> next;
-        for (int x = 0; x < ns; x++) {
-            const int j = redo[x];
-            const Chunk &k = chs[j];
-            rs.tb_rerun_cols += k.c1 - (flags[x] < 0 ? k.c0 : flags[x]);
-            if (flags[x] < 0 && !k.first) next.push_back(j - 1);
+        if (chs[j].last && !chs[j].first) {
+            int f = j;
+            while (!chs[f].first) f--;
+            ws.push_back(Walk{f, j - 1, oe[(size_t)j * K]});
         }
-        redo.swap(next);
+    std::vector<int> aj, ao, pj, pk, res;
+    while (true) {
+        pj.clear();
+        pk.clear();
+        for (Walk &w : ws) {
+            while (w.r >= w.f) {
+                const int o = match(w.r, w.b);
+                if (o < 0) break;
+                if (o > 0) {
+                    aj.push_back(w.r);
+                    ao.push_back(o);
+                }
+                w.b = oe[(size_t)w.r * K + o];
+                w.r--;
+            }
+            if (w.r < w.f) continue;
+            pj.push_back(w.r);
+            pk.push_back(w.b);
+            int g = oe[(size_t)w.r * K];
+            for (int j = w.r - 1; j >= w.f; j--) {
+                const int o = match(j, g);
+                if (o >= 0) {
+                    g = oe[(size_t)j * K + o];
+                    continue;
+                }
+                pj.push_back(j);
+                pk.push_back(g);
+                g = oe[(size_t)j * K];
+            }
+        }
+        if (pj.empty()) break;
+        const int np = (int)pj.size();
+        HIP_OK(hipMemcpyAsync(m.d_ridx, pj.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(m.d_rk1, pk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+        if (fp32 ? launch_lds(k_tb_rerun<float>, np, sht, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)m.d_ridx,
+                              (const int *)m.d_rk1, K, m.d_obuf, (const int *)m.d_ostart, m.d_rres)
+                 : launch_lds(k_tb_rerun<double>, np, sht, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)m.d_ridx,
+                              (const int *)m.d_rk1, K, m.d_obuf, (const int *)m.d_ostart, m.d_rres))
+            return 1;
+        res.resize(3 * (size_t)np);
+        HIP_OK(hipMemcpyAsync(res.data(), m.d_rres, sizeof(int) * res.size(), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        rs.tb_passes++;
+        for (int x = 0; x < np; x++) {
+            const int j = pj[x];
+            rs.tb_rerun_cols += chs[j].c1 - (res[3 * x] < 0 ? chs[j].c0 : res[3 * x]);
+            os[(size_t)j * K] = pk[x];
+            oe[(size_t)j * K] = res[3 * x + 2];
+        }
     }
-    // the full tile list again on the device (the passes overwrote its front)
-    HIP_OK(hipMemcpyAsync(m.d_chunks, chs.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
+    if (!aj.empty()) {
+        const int na = (int)aj.size();
+        HIP_OK(hipMemcpyAsync(m.d_aidx, aj.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(m.d_aopt, ao.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_tb_adopt, dim3(na), dim3(256), 0, s, (const DevInst *)m.dD,
+                           (const Chunk *)m.d_chunks, (const int *)m.d_aidx, (const int *)m.d_aopt,
+                           m.d_obuf);
+        HIP_OK(hipGetLastError());
+    }
+    rs.tb_adopted = (int)aj.size();
     return 0;
 }
 
 // The traceback of every instance whose forward table is resident: the
-// speculative tiles in one launch, then fix-up passes in which a tile is
-// re-sampled from its true boundary state until it coalesces with the
-// stored path; a tile that does not coalesce passes the re-run on to the
-// tile before it.
-static int traceback_stage(Batch::Impl &m, bool fp32, int tile_cols, int warm, bool fixup,
-                           RunStats &rs)
+// options of every tile in one launch, then the resolution (re-runs where
+// no option fits).  resolve = false leaves the speculative option-0 paths
+// (diagnostics).
+static int traceback_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt, RunStats &rs)
 {
     size_t sht;
-    if (plan_tiles(m, fp32, tile_cols, warm, sht)) return 1;
-    const int nch = (int)m.chunks.size();
-    hipStream_t s = m.st.main;
-    if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s)
-             : launch_tb<double>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s))
-        return 1;
-    return fixup ? fixup_passes(m, fp32, sht, rs) : 0;
+    const int K = std::max(1, std::min(opt.tile_opts, TB_KMAX));
+    if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
+    if (launch_options(m, fp32, sht)) return 1;
+    return opt.fixup ? resolve_tiles(m, fp32, sht, rs) : 0;
 }
 
-int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out)
+int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out, int warm,
+                      int opts)
 {
     DEVICE_LOCK;
     BatchOptions opt;
@@ -2207,7 +2472,12 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
     for (int i = 0; i < ncand; i++) {
         RunStats rs;
         (void)hipEventRecord(c.ev[6], c.main);
-        if (traceback_stage(m, fp32, cand[i], BatchOptions().tile_warm, true, rs)) return -1;
+        BatchOptions topt;
+        topt.prec = prec;
+        topt.tile_cols = cand[i];
+        topt.tile_warm = warm;
+        topt.tile_opts = opts;
+        if (traceback_stage(m, fp32, topt, rs)) return -1;
         (void)hipEventRecord(c.ev[7], c.main);
         HIP_OK(hipStreamSynchronize(c.main));
         float ms = 0;
@@ -2556,17 +2826,8 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     (void)hipEventRecord(c.ev[4], s);
     hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD, m.d_list);
     HIP_OK(hipGetLastError());
-    // speculative traceback tiles
-    size_t sht = 0;
-    if (opt.traceback) {
-        if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, sht)) return 1;
-        const int nch = (int)m.chunks.size();
-        if (fp32 ? launch_tb<float>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s)
-                 : launch_tb<double>(m.dD, m.d_chunks, nch, 0, nullptr, nullptr, sht, s))
-            return 1;
-    }
-
-    if (opt.traceback && opt.fixup && fixup_passes(m, fp32, sht, rs)) return 1;
+    // traceback tiles: the options of every tile, then their resolution
+    if (opt.traceback && traceback_stage(m, fp32, opt, rs)) return 1;
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));

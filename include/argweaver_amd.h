@@ -363,6 +363,12 @@ int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int tracebac
  * and the ms of the main forward launch. */
 int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback, int fixup,
                        int fw_tile, int fw_warm, double fw_tol, double *stats);
+/* Traceback tiles of later runs: warm-up columns of a tile's speculative
+ * chain (default 256) and options per tile, 1 .. 4 (default 3: the
+ * speculative chain and chains from the neighbouring states of its branch;
+ * a tile no option fits is re-run).  The sampled thread is the same for
+ * every setting.  0 on success, -1 for a value out of range. */
+int awamd_batch_set_traceback(void *batch, int warm, int options);
 /* Traceback-schedule autotuner: runs the forward once, times the traceback
  * for each candidate tile length (0: one tile per instance) and returns the
  * fastest (-1 on error); ms_out[ncand] receives the timings when given. */
