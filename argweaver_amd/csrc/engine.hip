@@ -45,6 +45,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -1042,14 +1043,15 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
     fw_load_block<Real, R, NT1>(d, b, fl, TMS);
 #pragma unroll
     for (int k = 0; k < 32 * GCOPY / 64; k++) GP[k * 64 + lane] = 0.0;
-    // Column classes of 64 columns at a time, one per lane, read with
-    // readlane; the next 64 are loaded a block ahead, so the one wait per
-    // 64 columns finds them loaded (vmcnt also counts the table stores, so
-    // a per-column load would wait for the previous column's stores).
+    // Column classes of a window of 64 columns, one per lane, read with
+    // readlane; the next window's are loaded a window ahead.  vmcnt counts
+    // the table stores too, so any wait on a load inside the column loop
+    // would also wait for the stores of the column before: the windows are
+    // an outer loop, and the one wait per window is taken there.
     const AW_GLOBAL int *clsg = gp(d.cls);
-    const int cb = c0 & ~63;
-    int clsv = clsg[min(cb + lane, L - 1)];
-    int clsn = clsg[min(cb + 64 + lane, L - 1)];
+    int wb = c0 & ~63;
+    int clsv = clsg[min(wb + lane, L - 1)];
+    int clsn = clsg[min(wb + 64 + lane, L - 1)];
     // Variant rows (row index = the column's class) are consumed in order
     // and staged in LDS, VST at a time, so that a refill costs a few waits
     // and the columns read their row from LDS.
@@ -1155,11 +1157,19 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
     }
     finish(c0, true);
 
-    for (int col = c0 + 1; col <= clast; col++) {
-        if ((col & 63) == 0) {
+    // an empty asm reading the register makes the compiler complete its
+    // load here rather than at a use inside the column loop
+    asm volatile("" : "+v"(clsv));
+    int col = c0 + 1;
+    while (col <= clast && !rejoined) {
+        if (col >= wb + 64) {
+            wb += 64;
             clsv = clsn;
-            clsn = clsg[min(col + 64 + lane, L - 1)];
+            asm volatile("" : "+v"(clsv));
+            clsn = clsg[min(wb + 64 + lane, L - 1)];
         }
+        const int wend = min(wb + 63, clast);
+    for (; col <= wend; col++) {
         const int cls = __builtin_amdgcn_readlane(clsv, col & 63);
         const bool var = cls >= 0;
         Real ev[R];
@@ -1276,6 +1286,7 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
         }
         finish(col, false);
         if (rejoined) break;
+    }
     }
     if (early && lane == 0) *early = rejoined ? 1 : 0;
 }
@@ -1806,19 +1817,22 @@ __device__ __forceinline__ void tb_rerun(const DevInst d, const Chunk &ch, int k
     }
 }
 
-// Launch of the speculative options: block (j, o) of tile j, option o sits
-// at 8 * (K * (j / 8) + o) + j % 8, so the options of a tile run on one XCD
-// (blocks go round-robin over the eight) and share its L2 for their rows.
+// Launch of the speculative options of the tiles list[0..nlist): block
+// (jj, o) of tile list[jj], option o sits at 8 * (K * (jj / 8) + o) + jj % 8,
+// so the options of a tile run on one XCD (blocks go round-robin over the
+// eight) and share its L2 for their rows.
 template <class Real>
 __global__ __launch_bounds__(64) void k_tb_spec(const DevInst *__restrict__ D,
-                                                const Chunk *__restrict__ chunks, int nch, int K,
+                                                const Chunk *__restrict__ chunks,
+                                                const int *__restrict__ list, int nlist, int K,
                                                 int *__restrict__ obuf, int *__restrict__ ostart,
                                                 int *__restrict__ oend)
 {
     extern __shared__ double tb_dyn[];
     const int bx = blockIdx.x;
-    const int j = (bx / (8 * K)) * 8 + bx % 8, o = (bx / 8) % K;
-    if (j >= nch) return;
+    const int jj = (bx / (8 * K)) * 8 + bx % 8, o = (bx / 8) % K;
+    if (jj >= nlist) return;
+    const int j = list[jj];
     const Chunk ch = chunks[j];
     const DevInst &d = D[ch.inst];
     int *os = ostart + (size_t)j * K + o, *oe = oend + (size_t)j * K + o;
@@ -1924,14 +1938,19 @@ std::recursive_mutex &device_mutex()
 // The stream and events of one batch.  Each batch owns its own stream (at
 // the default priority); batches never overlap on the device (see
 // device_mutex).
+// The batch's streams: main carries every stage; side carries only the
+// traceback tiles of the instances whose forward is final while the forward
+// re-runs of others continue on main.  The two are joined by the host
+// (stream synchronisation), never by a stream waiting on an event.
 struct Streams {
     bool init = false;
-    hipStream_t main = nullptr;
+    hipStream_t main = nullptr, side = nullptr;
     hipEvent_t ev[8];
     int create()
     {
         if (init) return 0;
         HIP_OK(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
         init = true;
         return 0;
@@ -1940,8 +1959,10 @@ struct Streams {
     {
         if (!init) return;
         (void)hipStreamSynchronize(main);
+        (void)hipStreamSynchronize(side);
         for (auto &e : ev) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(main);
+        (void)hipStreamDestroy(side);
         init = false;
     }
 };
@@ -2061,6 +2082,7 @@ struct Batch::Impl {
     std::vector<double> lnl;
     std::vector<int> nvar;
     std::vector<Chunk> chunks;
+    std::vector<int> tlist0, tlist1;   // host tile lists of the option launches
     DevBuf in, work, tab, aux, vbuf, tiles, fwb;
     Arena arena;   // base == nullptr: buffers come from hipMalloc
     std::vector<std::vector<float>> staged32;   // FP32 planes of host-prepared instances
@@ -2072,6 +2094,7 @@ struct Batch::Impl {
     // re-run lists and results, adopted alternatives
     int *d_ostart = nullptr, *d_oend = nullptr, *d_obuf = nullptr;
     int *d_ridx = nullptr, *d_rk1 = nullptr, *d_rres = nullptr, *d_aidx = nullptr, *d_aopt = nullptr;
+    int *d_tlist = nullptr;   // [2 nch] tile lists of the two option launches
     int tb_opts = 1;
     Chunk *d_chunks = nullptr;
     FwTile *d_fwt = nullptr, *d_fwre = nullptr;
@@ -2321,6 +2344,7 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
         m.d_rres = p.take<int>(3 * (size_t)nch);
         m.d_aidx = p.take<int>(nch);
         m.d_aopt = p.take<int>(nch);
+        m.d_tlist = p.take<int>(2 * (size_t)nch);
         m.d_obuf = p.take<int>((size_t)std::max(1LL, (long long)(K - 1) * ocols));
         if (pass == 0 && m.tiles.ensure(p.used + 4096)) return 1;
     }
@@ -2329,16 +2353,28 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
     return 0;
 }
 
-// The speculative launch: every option of every tile.
-static int launch_options(Batch::Impl &m, bool fp32, size_t sht)
+// The speculative launch on stream s: every option of the tiles of the
+// instances with take[inst] set (all when take is null); slot 0 or 1 of the
+// device tile lists.
+static int launch_options(Batch::Impl &m, bool fp32, size_t sht, const std::vector<char> *take,
+                          int slot, hipStream_t s)
 {
     const int nch = (int)m.chunks.size(), K = m.tb_opts;
-    const int grid = (nch + 7) / 8 * 8 * K;
-    hipStream_t s = m.st.main;
+    std::vector<int> &list = slot ? m.tlist1 : m.tlist0;
+    list.clear();
+    for (int j = 0; j < nch; j++)
+        if (!take || (*take)[m.chunks[j].inst]) list.push_back(j);
+    const int nl = (int)list.size();
+    if (!nl) return 0;
+    int *dl = m.d_tlist + (size_t)slot * nch;
+    HIP_OK(hipMemcpyAsync(dl, list.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
+    const int grid = (nl + 7) / 8 * 8 * K;
     return fp32 ? launch_lds(k_tb_spec<float>, grid, sht, s, (const DevInst *)m.dD,
-                             (const Chunk *)m.d_chunks, nch, K, m.d_obuf, m.d_ostart, m.d_oend)
+                             (const Chunk *)m.d_chunks, (const int *)dl, nl, K, m.d_obuf,
+                             m.d_ostart, m.d_oend)
                 : launch_lds(k_tb_spec<double>, grid, sht, s, (const DevInst *)m.dD,
-                             (const Chunk *)m.d_chunks, nch, K, m.d_obuf, m.d_ostart, m.d_oend);
+                             (const Chunk *)m.d_chunks, (const int *)dl, nl, K, m.d_obuf,
+                             m.d_ostart, m.d_oend);
 }
 
 // Resolution of the tiles, right to left within each instance.  A tile
@@ -2452,7 +2488,7 @@ static int traceback_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt, R
     size_t sht;
     const int K = std::max(1, std::min(opt.tile_opts, TB_KMAX));
     if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
-    if (launch_options(m, fp32, sht)) return 1;
+    if (launch_options(m, fp32, sht, nullptr, 0, m.st.main)) return 1;
     return opt.fixup ? resolve_tiles(m, fp32, sht, rs) : 0;
 }
 
@@ -2530,7 +2566,8 @@ inline double fw_cost_model(int R)
 // fail are re-run from that column until every check passes (a re-run
 // changes the boundary column of the next tile, which is checked again).
 static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
-                         const std::vector<int> &order, size_t shf, int t1max, RunStats &rs)
+                         const std::vector<int> &order, size_t shf, int t1max, RunStats &rs,
+                         const std::function<int(const std::vector<char> &)> &first_check)
 {
     Streams &c = m.st;
     hipStream_t s = c.main;
@@ -2599,6 +2636,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
     for (int k = 0; k < nt; k++)
         if (tl[k].bin >= 0) chk.push_back(k);
     std::vector<int> flags;
+    bool first = true;
     while (!chk.empty()) {
         const int nc = (int)chk.size();
         HIP_OK(hipMemcpyAsync(m.d_fidx, chk.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
@@ -2615,6 +2653,13 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         std::vector<int> redo;
         for (int x = 0; x < nc; x++)
             if (flags[x]) redo.push_back(chk[x]);
+        if (first) {
+            // the instances that will be re-run; the others are final
+            first = false;
+            std::vector<char> busy(m.h.size(), 0);
+            for (int k : redo) busy[tl[k].inst] = 1;
+            if (first_check(busy)) return 1;
+        }
         if (redo.empty()) break;
         // re-run from the boundary column, then check the tiles after them
         std::vector<FwTile> sub;
@@ -2644,6 +2689,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         for (int x = 0; x < nr; x++)
             if (!early[x] && next[redo[x]] >= 0) chk.push_back(next[redo[x]]);
     }
+    if (first) return first_check(std::vector<char>(m.h.size(), 0));
     return 0;
 }
 
@@ -2822,12 +2868,35 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         shf = std::max(shf, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
     }
     RunStats rs;
-    if (forward_stage(m, fp32, opt, order, shf, t1max, rs)) return 1;
+    // Once the first check of the forward tiles has shown which instances
+    // are re-run, the traceback options of all the others go to the side
+    // stream and run while the re-runs continue on the main stream.
+    size_t sht = 0;
+    std::vector<char> busy;
+    const int K = std::max(1, std::min(opt.tile_opts, TB_KMAX));
+    auto early_tb = [&](const std::vector<char> &b) -> int {
+        if (!opt.traceback || std::find(b.begin(), b.end(), 1) == b.end()) return 0;
+        busy = b;
+        if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
+        HIP_OK(hipStreamSynchronize(s));   // the tile list is on the device
+        std::vector<char> done(n);
+        for (int i = 0; i < n; i++) done[i] = !b[i];
+        return launch_options(m, fp32, sht, &done, 0, c.side);
+    };
+    if (forward_stage(m, fp32, opt, order, shf, t1max, rs, early_tb)) return 1;
     (void)hipEventRecord(c.ev[4], s);
     hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD, m.d_list);
     HIP_OK(hipGetLastError());
     // traceback tiles: the options of every tile, then their resolution
-    if (opt.traceback && traceback_stage(m, fp32, opt, rs)) return 1;
+    if (opt.traceback) {
+        if (busy.empty()) {
+            if (traceback_stage(m, fp32, opt, rs)) return 1;
+        } else {
+            if (launch_options(m, fp32, sht, &busy, 1, s)) return 1;
+            HIP_OK(hipStreamSynchronize(c.side));
+            if (opt.fixup && resolve_tiles(m, fp32, sht, rs)) return 1;
+        }
+    }
     (void)hipEventRecord(c.ev[5], s);
     for (int i = 0; i < n; i++)
         HIP_OK(hipMemcpyAsync(&m.lnl[i], m.h[i].lnl, sizeof(double), hipMemcpyDeviceToHost, s));

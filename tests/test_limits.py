@@ -25,3 +25,13 @@ def test_batch_takes_instance_at_node_limit(aw):
         # over 1024 states in a block is the other limit
         pytest.skip("instance exceeds the states-per-block limit")
     aw.delete_local_trees(tr)
+
+
+def test_traceback_options_out_of_range(aw):
+    # options per traceback tile: 1 .. 4; warm-up columns >= 0
+    b = aw.Batch()
+    b.set_traceback(0, 1)
+    b.set_traceback(4096, 4)
+    for bad in ((256, 0), (256, 5), (-1, 3)):
+        with pytest.raises(ValueError):
+            b.set_traceback(*bad)
