@@ -217,6 +217,9 @@ awamd_server_submit = _export(
 awamd_server_wait = _export("awamd_server_wait", c_int,
                             [c_void_p, c_int, c_double, c_int_p, c_double_p, c_double_p])
 awamd_server_now = _export("awamd_server_now", c_double, [c_void_p])
+awamd_server_new_ex = _export("awamd_server_new_ex", c_void_p, [c_double, c_int, c_int, c_int, c_int])
+awamd_server_wait_ex = _export("awamd_server_wait_ex", c_int,
+                               [c_void_p, c_int, c_double, c_int_p, c_double_p, c_double_p])
 
 _libc = ctypes.CDLL(None)
 
@@ -884,11 +887,13 @@ class Batch(object):
 
 class Server(object):
     """Continuous-batching server (awamd_server_*): submit() from any thread,
-    wait() for a result; the requests queued while a batch runs are admitted
-    together, in arrival order, within the device-memory budget."""
+    wait() for a result.  Each of `drivers` driver threads runs a batch of
+    its own: the requests queued while one batch is on the GPU are admitted
+    together, in arrival order, by a free driver within its share of the
+    device-memory budget, and run beside the batch in flight."""
 
-    def __init__(self, hbm_budget_gb=64.0, precision=1, tile_cols=2000, max_batch=64):
-        self.ptr = awamd_server_new(hbm_budget_gb, precision, tile_cols, max_batch)
+    def __init__(self, hbm_budget_gb=64.0, precision=1, tile_cols=2000, max_batch=64, drivers=2):
+        self.ptr = awamd_server_new_ex(hbm_budget_gb, precision, tile_cols, max_batch, drivers)
         if not self.ptr:
             raise RuntimeError("awamd_server_new failed")
         self.L = {}
@@ -911,9 +916,9 @@ class Server(object):
         L = self.L[i]
         path = np.zeros(L, np.int32)
         lnl = c_double(0)
-        t = np.zeros(3, np.float64)
-        rc = awamd_server_wait(self.ptr, i, float(timeout or 0.0), _iptr(path), ctypes.byref(lnl),
-                               _dptr(t))
+        t = np.zeros(5, np.float64)
+        rc = awamd_server_wait_ex(self.ptr, i, float(timeout or 0.0), _iptr(path),
+                                  ctypes.byref(lnl), _dptr(t))
         if rc == -2:
             raise TimeoutError("request %d not done after %.1f s" % (i, timeout))
         del self.L[i]
@@ -921,7 +926,8 @@ class Server(object):
             raise ValueError("request %d refused: over the kernels' limits" % i)
         if rc != 0:
             raise RuntimeError("request %d failed (%d)" % (i, rc))
-        return dict(path=path, lnl=lnl.value, t_submit=t[0], t_start=t[1], t_done=t[2])
+        return dict(path=path, lnl=lnl.value, t_submit=t[0], t_start=t[1], t_done=t[2],
+                    batch_size=int(t[3]), driver=int(t[4]))
 
     def now(self):
         return awamd_server_now(self.ptr)

@@ -940,15 +940,21 @@ extern "C" LocalTrees *awamd_simulate(int k, int seqlen, const double *times, in
 
 extern "C" {
 
-void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch)
+void *awamd_server_new_ex(double hbm_budget_gb, int precision, int tile_cols, int max_batch,
+                          int drivers)
 {
     if (!device_available()) {
         fprintf(stderr, "argweaver_amd: awamd_server_new needs a HIP device\n");
         return nullptr;
     }
+    if (drivers < 1 || drivers > 4) {
+        fprintf(stderr, "argweaver_amd: %d server drivers (1 .. 4)\n", drivers);
+        return nullptr;
+    }
     ServeOptions o;
     o.hbm_budget = (size_t)(hbm_budget_gb * 1073741824.0);
     o.max_batch = max_batch;
+    o.drivers = drivers;
     o.run.prec = precision == 1 ? Precision::FP32 : Precision::FP64;
     o.run.tile_cols = tile_cols;
     // FP32 requests longer than 50000 columns run as forward tiles
@@ -958,6 +964,11 @@ void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int m
         o.run.fw_tol = 1e-4;
     }
     return new Server(o);
+}
+
+void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch)
+{
+    return awamd_server_new_ex(hbm_budget_gb, precision, tile_cols, max_batch, ServeOptions().drivers);
 }
 
 void awamd_server_free(void *server) { delete reinterpret_cast<Server *>(server); }
@@ -986,6 +997,25 @@ int awamd_server_wait(void *server, int id, double timeout_s, int *path, double 
         times3[0] = r.t_submit;
         times3[1] = r.t_start;
         times3[2] = r.t_done;
+    }
+    return r.status;
+}
+
+int awamd_server_wait_ex(void *server, int id, double timeout_s, int *path, double *lnl,
+                         double *info5)
+{
+    ServeResult r;
+    const int w = reinterpret_cast<Server *>(server)->wait(id, &r, timeout_s);
+    if (w == 2) return -2;
+    if (w) return -1;
+    if (path) std::copy(r.path.begin(), r.path.end(), path);
+    if (lnl) *lnl = r.lnl;
+    if (info5) {
+        info5[0] = r.t_submit;
+        info5[1] = r.t_start;
+        info5[2] = r.t_done;
+        info5[3] = r.batch_size;
+        info5[4] = r.driver;
     }
     return r.status;
 }

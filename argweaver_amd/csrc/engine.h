@@ -160,6 +160,10 @@ public:
     void use_workspace(void *base, size_t cap);
     // create the batch's stream and events now (else the first upload does)
     int init_streams();
+    // false (default): the batch's device work takes the process-wide lock
+    // and never overlaps another batch's; true: it takes a lock of its own
+    // and may run beside other batches (the server's drivers)
+    void set_concurrent(bool on);
     // the instance's row pointers and uniforms must stay valid until upload()
     int add(const HostInstance &hi, const double *u, const double *prior = nullptr);
     // drop the instances, keep the device buffers for the next set

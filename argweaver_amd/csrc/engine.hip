@@ -1923,11 +1923,15 @@ int ctx_init()
     return 0;
 }
 
-// Device work of all batches is serialised by one process-wide lock: the
-// upload, run, tuning and copy-out methods of Batch take it, so batches
-// driven from different host threads run one after another, and the
-// per-launch hipFuncSetAttribute calls on the shared kernel symbols never
-// race.  Recursive because run() uploads a batch that is not yet uploaded.
+// The upload, run, tuning and copy-out methods of a Batch hold a lock for
+// their whole device work.  By default it is one process-wide lock, so
+// batches driven from different host threads run one after another; a
+// batch set concurrent (the server's) holds a lock of its own and runs
+// beside the others.  Concurrent batches are safe because no stream ever
+// waits on another stream's event (streams are joined by the host), so no
+// hardware queue can hold a wait ahead of the work it waits for, and the
+// kernels' dynamic-LDS limits only rise (set_lds_limit).  Recursive because
+// run() uploads a batch that is not yet uploaded.
 std::recursive_mutex &device_mutex()
 {
     static std::recursive_mutex mu;
@@ -1935,9 +1939,6 @@ std::recursive_mutex &device_mutex()
 }
 #define DEVICE_LOCK std::lock_guard<std::recursive_mutex> device_lock_(device_mutex())
 
-// The stream and events of one batch.  Each batch owns its own stream (at
-// the default priority); batches never overlap on the device (see
-// device_mutex).
 // The batch's streams: main carries every stage; side carries only the
 // traceback tiles of the instances whose forward is final while the forward
 // re-runs of others continue on main.  The two are joined by the host
@@ -2048,13 +2049,28 @@ int fw_lds_reals_r(bool fp32, int R)
     }
 }
 
+// The dynamic-LDS limit of a kernel only ever rises, under one lock: a
+// batch on another host thread that set a larger limit for the same kernel
+// is never undercut between its attribute call and its launch.
+int set_lds_limit(const void *kern, size_t sh)
+{
+    static std::mutex mu;
+    static std::map<const void *, size_t> cur;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t &c = cur[kern];
+    if (sh <= c) return 0;
+    HIP_OK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    c = sh;
+    return 0;
+}
+
 template <class Real>
 int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double *bnorm,
               size_t sh, hipStream_t st, int T1max, double tol = 0.0, int *early = nullptr)
 {
     auto kern = early ? ((T1max <= 19) ? k_fw_all<Real, 19, true> : k_fw_all<Real, MAXT - 1, true>)
                       : ((T1max <= 19) ? k_fw_all<Real, 19, false> : k_fw_all<Real, MAXT - 1, false>);
-    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    if (set_lds_limit((const void *)kern, sh)) return 1;
     hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, tiles, (Real *)bnd, bnorm, tol, early);
     HIP_OK(hipGetLastError());
     return 0;
@@ -2064,7 +2080,7 @@ int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double
 template <class Kern, class... Args>
 int launch_lds(Kern kern, int grid, size_t sh, hipStream_t st, Args... args)
 {
-    HIP_OK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    if (set_lds_limit((const void *)kern, sh)) return 1;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64), sh, st, args...);
     HIP_OK(hipGetLastError());
     return 0;
@@ -2103,6 +2119,8 @@ struct Batch::Impl {
     int *d_fidx = nullptr, *d_fflag = nullptr;
     bool uploaded = false;
     Precision prec_done = Precision::FP64;
+    bool own_lock = false;     // set_concurrent: this batch's own lock
+    std::recursive_mutex mu;
 
     ~Impl()
     {
@@ -2117,7 +2135,12 @@ struct Batch::Impl {
     }
 };
 
+// the lock of a batch's device work (see device_mutex)
+#define BATCH_LOCK \
+    std::lock_guard<std::recursive_mutex> batch_lock_(impl_->own_lock ? impl_->mu : device_mutex())
+
 Batch::Batch() : impl_(new Impl) {}
+void Batch::set_concurrent(bool on) { impl_->own_lock = on; }
 Batch::~Batch() { delete impl_; }
 int Batch::size() const { return (int)impl_->hosts.size(); }
 
@@ -2201,7 +2224,7 @@ int Batch::add(const HostInstance &hi, const double *u, const double *prior)
 // Inputs: trees, sequences, uniforms and the model of every instance.
 int Batch::upload()
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     if (ctx_init()) return 1;
     Impl &m = *impl_;
     if (m.st.create()) return 1;
@@ -2495,7 +2518,7 @@ static int traceback_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt, R
 int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out, int warm,
                       int opts)
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     BatchOptions opt;
     opt.prec = prec;
     opt.traceback = false;
@@ -2529,7 +2552,7 @@ int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out
 
 int Batch::tune_forward(const int *tiles, const int *warms, int ncand, double tol, double *ms_out)
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     int best = -1;
     double best_ms = 0;
     for (int i = 0; i < ncand; i++) {
@@ -2695,7 +2718,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
 
 int Batch::run(const BatchOptions &opt, RunStats *st)
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     if (ctx_init()) return 1;
     Impl &m = *impl_;
     if (!m.uploaded && upload()) return 1;
@@ -2925,7 +2948,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
 
 int Batch::copy_path(int i, int *path) const
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     const Impl &m = *impl_;
     HIP_OK(hipMemcpy(path, m.h[i].path, sizeof(int) * m.h[i].L, hipMemcpyDeviceToHost));
     return 0;
@@ -2933,7 +2956,7 @@ int Batch::copy_path(int i, int *path) const
 
 int Batch::copy_norms(int i, double *norms) const
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     const Impl &m = *impl_;
     HIP_OK(hipMemcpy(norms, m.h[i].norms, sizeof(double) * m.h[i].L, hipMemcpyDeviceToHost));
     return 0;
@@ -2943,7 +2966,7 @@ int Batch::copy_norms(int i, double *norms) const
 // reference's table layout) of the lane-run table
 int Batch::copy_alpha(int i, double *alpha) const
 {
-    DEVICE_LOCK;
+    BATCH_LOCK;
     const Impl &m = *impl_;
     const DevInst &d = m.h[i];
     const FlatTrees &ft = m.hosts[i].ft;

@@ -1,6 +1,7 @@
 // serve.cpp -- continuous batching of threading requests (see serve.h).
 #include "serve.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 
@@ -15,7 +16,8 @@ static double wall_seconds()
 Server::Server(const ServeOptions &opt) : opt_(opt), t0_(wall_seconds())
 {
     if (opt_.max_batch < 1) opt_.max_batch = 1;
-    driver_ = std::thread(&Server::worker, this);
+    opt_.drivers = std::max(1, std::min(opt_.drivers, 4));
+    for (int i = 0; i < opt_.drivers; i++) drivers_.emplace_back(&Server::worker, this, i);
 }
 
 Server::~Server()
@@ -25,7 +27,7 @@ Server::~Server()
         stop_ = true;
     }
     cv_work_.notify_all();
-    driver_.join();
+    for (std::thread &t : drivers_) t.join();
     for (Request *r : queue_) delete r;
 }
 
@@ -126,9 +128,13 @@ int Server::wait(int id, ServeResult *out, double timeout_s)
     return 0;
 }
 
-void Server::worker()
+void Server::worker(int index)
 {
+    // each driver's batch takes its own device lock, so the drivers' batches
+    // run side by side on the device
     Batch batch;
+    batch.set_concurrent(true);
+    const size_t budget = opt_.hbm_budget / opt_.drivers;
     std::vector<Request *> mine;
     while (true) {
         {
@@ -139,7 +145,7 @@ void Server::worker()
             size_t taken = 0;
             while (!queue_.empty() && (int)mine.size() < opt_.max_batch) {
                 const size_t b = queue_.front()->bytes;
-                if (!mine.empty() && taken + b > opt_.hbm_budget) break;
+                if (!mine.empty() && taken + b > budget) break;
                 taken += b;
                 mine.push_back(queue_.front());
                 queue_.pop_front();
@@ -159,6 +165,7 @@ void Server::worker()
             o.t_submit = r->t_submit;
             o.t_start = t_start;
             o.batch_size = (int)mine.size();
+            o.driver = index;
             o.W = batch.W((int)i);
             if (!rc) {
                 o.lnl = batch.lnl((int)i);

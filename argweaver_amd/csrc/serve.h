@@ -2,13 +2,14 @@
 //
 // Requests (one threading instance each: the ARG with one lineage removed,
 // the sequences, the uniforms of the traceback) are submitted from any host
-// thread.  One driver thread owns the device work: whenever the GPU is free
-// it takes every queued request, in arrival order, whose estimated HBM
-// footprint still fits the budget (a request larger than the whole budget
-// runs alone) and runs them as one batch, so the requests that arrived while
-// a batch ran share the next launch.  All HIP calls of a server come from
-// its driver thread.  Each request's result does not depend on the batch it
-// ran in.
+// thread.  Driver threads own the device work, each with a batch of its own:
+// a driver that is free takes every queued request, in arrival order, whose
+// estimated HBM footprint still fits its share of the budget (a request
+// larger than the share runs alone) and runs them as one batch.  With two
+// drivers, requests that arrive while a batch is on the GPU are admitted by
+// the other driver at once and their work overlaps on the device with the
+// batch in flight.  All HIP calls of a server come from its driver threads.
+// Each request's result does not depend on the batch it ran in.
 #pragma once
 
 #include <condition_variable>
@@ -25,8 +26,9 @@
 namespace aw {
 
 struct ServeOptions {
-    size_t hbm_budget = size_t(64) << 30;  // bytes of device memory for one batch
+    size_t hbm_budget = size_t(64) << 30;  // bytes of device memory, shared by the drivers
     int max_batch = 64;                    // requests per batch
+    int drivers = 2;                       // batches in flight at once (1 .. 4)
     BatchOptions run;                      // precision and traceback tiles
 };
 
@@ -42,6 +44,7 @@ struct ServeResult {
     long long W = 0;         // column-states of the request
     double t_submit = 0, t_start = 0, t_done = 0;   // seconds on the server clock
     int batch_size = 0;
+    int driver = 0;          // the driver thread that ran it
 };
 
 class Server {
@@ -71,7 +74,7 @@ private:
         size_t bytes;
         double t_submit;
     };
-    void worker();
+    void worker(int index);
 
     ServeOptions opt_;
     std::mutex mu_;
@@ -80,7 +83,7 @@ private:
     std::map<int, ServeResult> done_;
     int next_id_ = 0;
     bool stop_ = false;
-    std::thread driver_;
+    std::vector<std::thread> drivers_;
     double t0_;
 };
 

@@ -392,12 +392,13 @@ LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
                            const double *popsizes, double rho, double mu,
                            unsigned long long seed, char *seqs);
 
-/* Continuous-batching server (no counterpart in the reference).  One
- * driver thread owns the device work: whenever the GPU is free it takes the
- * queued requests, in arrival order, while their estimated device footprint
- * fits hbm_budget_gb (at most max_batch; a request larger than the budget
- * runs alone) and runs them as one batch, so requests that arrive while a
- * batch runs share the next launch.  submit copies the sequence rows and
+/* Continuous-batching server (no counterpart in the reference).  Driver
+ * threads (two by default) own the device work, each with a batch of its
+ * own: a free driver takes the queued requests, in arrival order, while
+ * their estimated device footprint fits its share of hbm_budget_gb (at most
+ * max_batch; a request larger than the share runs alone) and runs them as
+ * one batch, so requests that arrive while a batch is on the GPU are
+ * admitted by the other driver and run beside it.  submit copies the sequence rows and
  * uniforms and returns an id (-1 for a time grid of more than 32 points,
  * which is not queued); wait blocks for it (at most timeout_s seconds
  * when timeout_s > 0) and fills path[seqlen], the log-likelihood and
@@ -410,12 +411,19 @@ LocalTrees *awamd_simulate(int k, int seqlen, const double *times, int ntimes,
  * run requests longer than 50000 columns as forward tiles (as
  * awamd_batch_run_ex with fw_tile 50000, fw_warm 20000, fw_tol 1e-4). */
 void *awamd_server_new(double hbm_budget_gb, int precision, int tile_cols, int max_batch);
+/* the same with the number of driver threads, 1 .. 4 (NULL outside) */
+void *awamd_server_new_ex(double hbm_budget_gb, int precision, int tile_cols, int max_batch,
+                          int drivers);
 void awamd_server_free(void *server);
 int awamd_server_submit(void *server, LocalTrees *trees, const double *times, int ntimes,
                         const double *popsizes, double rho, double mu, char **seqs,
                         int new_chrom, const double *uniforms);
 int awamd_server_wait(void *server, int id, double timeout_s, int *path, double *lnl,
                       double *times3);
+/* awamd_server_wait with info5 = {submitted, started, finished, requests in
+ * its batch, driver index} */
+int awamd_server_wait_ex(void *server, int id, double timeout_s, int *path, double *lnl,
+                         double *info5);
 double awamd_server_now(void *server);
 
 #ifdef __cplusplus

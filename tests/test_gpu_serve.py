@@ -1,5 +1,6 @@
-"""Serving: requests submitted while a batch runs are admitted together into
-the next batch; every result equals the same request run alone."""
+"""Serving: requests submitted while a batch runs are admitted by a free
+driver and run beside it on the device; every result equals the same request
+run alone."""
 import threading
 import time
 
@@ -68,10 +69,39 @@ def test_budget_admission(aw):
     for it, r in zip(items, res):
         path, lnl = _alone(aw, it, 1, 0)
         assert r["lnl"] == lnl and np.array_equal(r["path"], path)
-    # one at a time: no two requests overlap in time
-    spans = sorted((r["t_start"], r["t_done"]) for r in res)
-    for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
-        assert s1 >= e0 - 1e-6
+    # one request per batch, and at most one batch per driver at a time
+    assert all(r["batch_size"] == 1 for r in res)
+    for r in res:
+        t = r["t_start"] + 1e-6
+        assert sum(1 for q in res if q["t_start"] <= t < q["t_done"]) <= 2
+
+
+def test_admitted_mid_flight_equals_alone(aw):
+    """A short request submitted while a long one is on the GPU is admitted
+    by the other driver at once, runs and finishes while the long one is
+    still running, and both results equal the requests run alone."""
+    print("serve mid-flight admission", flush=True)
+    long_tr, long_seqs, _, long_model = instance(aw, 40, 200000, 91)
+    long_u = np.random.RandomState(91).uniform(size=200000)
+    short = _items(aw)[0]
+    srv = aw.Server(hbm_budget_gb=16.0, precision=0, tile_cols=0, max_batch=8, drivers=2)
+    il = srv.submit(long_tr, long_seqs, *long_model, 39, long_u)
+    time.sleep(0.05)
+    i_s = srv.submit(short["trees"], short["seqs"], *short["model"], short["k"] - 1, short["u"])
+    rs = srv.wait(i_s, timeout=120)
+    rl = srv.wait(il, timeout=120)
+    srv.close()
+    print("long %.3f-%.3f driver %d, short %.3f-%.3f driver %d" % (
+        rl["t_start"], rl["t_done"], rl["driver"], rs["t_start"], rs["t_done"], rs["driver"]),
+        flush=True)
+    assert rs["driver"] != rl["driver"]
+    assert rl["t_start"] <= rs["t_start"] < rs["t_done"] < rl["t_done"]
+    path, lnl = _alone(aw, short, 0, 0)
+    assert rs["lnl"] == lnl and np.array_equal(rs["path"], path)
+    it = dict(k=40, trees=long_tr, seqs=long_seqs, model=long_model, u=long_u)
+    path, lnl = _alone(aw, it, 0, 0)
+    assert rl["lnl"] == lnl and np.array_equal(rl["path"], path)
+    aw.delete_local_trees(long_tr)
 
 
 def test_oversized_request_fails_alone(aw):
