@@ -203,6 +203,8 @@ awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p,
 awamd_batch_tune_forward = _export(
     "awamd_batch_tune_forward", c_int, [c_void_p, c_int_p, c_int_p, c_int, c_double, c_double_p])
 awamd_batch_path = _export("awamd_batch_path", c_int, [c_void_p, c_int, c_int_p])
+awamd_batch_alpha_range = _export("awamd_batch_alpha_range", c_int,
+                                  [c_void_p, c_int, c_int, c_int, c_double_p])
 awamd_batch_alpha = _export("awamd_batch_alpha", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_norms = _export("awamd_batch_norms", c_int, [c_void_p, c_int, c_double_p])
 awamd_batch_lnl = _export("awamd_batch_lnl", c_double, [c_void_p, c_int])
@@ -866,6 +868,15 @@ class Batch(object):
     def alpha(self, i):
         out = np.zeros(int(awamd_batch_W(self.ptr, i)), np.float64)
         awamd_batch_alpha(self.ptr, i, _dptr(out))
+        return out
+
+    def alpha_range(self, i, c0, c1, nstates):
+        """Columns [c0, c1) of instance i's table (nstates: the per-column
+        state counts of the instance, as get_nstates gives them)."""
+        n = int(np.sum(nstates[c0:c1]))
+        out = np.zeros(n, np.float64)
+        if awamd_batch_alpha_range(self.ptr, i, int(c0), int(c1), _dptr(out)) != 0:
+            raise ValueError("bad column range")
         return out
 
     def norms(self, i):

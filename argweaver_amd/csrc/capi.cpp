@@ -882,6 +882,11 @@ int awamd_batch_alpha(void *b, int i, double *alpha)
     return reinterpret_cast<AwBatch *>(b)->batch.copy_alpha(i, alpha);
 }
 
+int awamd_batch_alpha_range(void *b, int i, int c0, int c1, double *alpha)
+{
+    return reinterpret_cast<AwBatch *>(b)->batch.copy_alpha_range(i, c0, c1, alpha);
+}
+
 int awamd_batch_norms(void *b, int i, double *norms)
 {
     return reinterpret_cast<AwBatch *>(b)->batch.copy_norms(i, norms);

@@ -185,6 +185,8 @@ public:
                      double *ms_out = nullptr);
     int copy_path(int i, int *path) const;
     int copy_alpha(int i, double *alpha) const;   // block-major FP64 view of the table
+    // columns [c0, c1) of it (S of the column's block per column)
+    int copy_alpha_range(int i, int c0, int c1, double *alpha) const;
     int copy_norms(int i, double *norms) const;
     double lnl(int i) const;
     long long W(int i) const;

@@ -2994,6 +2994,36 @@ int Batch::copy_alpha(int i, double *alpha) const
     return 0;
 }
 
+int Batch::copy_alpha_range(int i, int c0, int c1, double *alpha) const
+{
+    BATCH_LOCK;
+    const Impl &m = *impl_;
+    const DevInst &d = m.h[i];
+    const FlatTrees &ft = m.hosts[i].ft;
+    if (c0 < 0 || c1 > d.L || c0 >= c1) return 1;
+    const int R = d.R, RW = 64 * R;
+    const size_t cells = (size_t)(c1 - c0) * RW;
+    std::vector<double> tab(cells);
+    if (m.prec_done == Precision::FP64) {
+        HIP_OK(hipMemcpy(tab.data(), (const double *)d.alpha + (size_t)c0 * RW,
+                         sizeof(double) * cells, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t32(cells);
+        HIP_OK(hipMemcpy(t32.data(), (const float *)d.alpha + (size_t)c0 * RW,
+                         sizeof(float) * cells, hipMemcpyDeviceToHost));
+        for (size_t x = 0; x < cells; x++) tab[x] = t32[x];
+    }
+    long long o = 0;
+    int b = 0;
+    for (int c = c0; c < c1; c++) {
+        while (ft.start[b] + ft.len[b] <= c) b++;
+        const int S = m.Sv[i][b];
+        const double *row = &tab[(size_t)(c - c0) * RW];
+        for (int k = 0; k < S; k++) alpha[o++] = row[lane_pos(k, R)];
+    }
+    return 0;
+}
+
 long long instance_states(const FlatTrees &ft, int T, std::vector<int> *Sv)
 {
     long long W = 0;

@@ -48,55 +48,115 @@ def make_workload(A, n, seed, kmin=8, kmax=64, lmin=5000, lmax=200000):
         full = A.trees_arrays(tr)   # the ARG before removal, for the CPU baseline
         A.remove_thread(tr, k - 1)
         u = rng.uniform(size=L)
-        insts.append(dict(k=k, L=L, trees=tr, full=full, seqs=seqs, u=u))
+        W, _ = A.num_column_states(tr, 20)
+        insts.append(dict(k=k, L=L, trees=tr, full=full, seqs=seqs, u=u, W=W))
     return insts, (times, pops, rho, mu)
 
 
-def cpu_baseline(A, insts, model, budget_s=15.0):
-    """Single-core CPU time of the same work on a bounded sample: the
-    reference library compiled from its sources (oracle/_ref) threading
-    instances with arghmm_sample_thread, or the oracle restatement when the
-    reference build is absent."""
+def _cpu_order(insts, seed):
+    """Instances in a W-weighted random order (without replacement): the
+    sample follows where the workload's column-states are."""
+    W = np.array([float(it["W"]) for it in insts])
+    rng = np.random.RandomState(seed + 5)
+    return list(rng.choice(len(insts), size=len(insts), replace=False, p=W / W.sum()))
+
+
+def _port_time(it, model):
+    """Oracle forward + stochastic traceback of one instance on one core:
+    (column-states, forward seconds, traceback seconds)."""
     import oracle_bind as O
+    import argweaver_amd.argweaverc as A
     times, pops, rho, mu = model
-    order = sorted(range(len(insts)), key=lambda i: insts[i]["L"] * insts[i]["k"])
+    P, Ag, S, bl = A.trees_arrays(it["trees"])
+    maps, seqids = A.trees_internal(it["trees"])
+    inst = O.Instance(times, pops, rho, mu, P, Ag, S, bl, it["seqs"], seqids=seqids,
+                      new_chrom=it["k"] - 1, mappings=maps)
+    t0 = time.perf_counter()
+    fw, _ = inst.forward()
+    t1 = time.perf_counter()
+    inst.traceback(fw, it["u"])
+    t2 = time.perf_counter()
+    return it["W"], t1 - t0, t2 - t1
+
+
+def _cpu_worker(args):
+    """One core's share of the aggregate: instances until `budget` seconds
+    of forward + traceback."""
+    insts, model, idx, budget = args
     W = 0
-    t_used = 0.0
+    t = 0.0
+    for i in idx:
+        w, f, b = _port_time(insts[i], model)
+        W += w
+        t += f + b
+        if t >= budget:
+            break
+    return W, t
+
+
+def cpu_baseline(A, insts, model, seed, budget_s=10.0, cores=16):
+    """The forward and stochastic traceback of the workload on the host
+    cores, on a W-weighted random sample: the oracle's C restatement (the
+    reference's algorithm, forward + traceback only) on one core, and the
+    aggregate of `cores` processes each running its own share in parallel.
+    The compiled reference's forward (arghmm_forward_alg, oracle/_ref) is
+    timed on the same instances, when built, to show the restatement's
+    speed against it.  Runs before the process touches the GPU (the pool
+    is forked)."""
+    import multiprocessing as mp
+    import oracle_bind as O
+    order = _cpu_order(insts, seed)
+    # one core
+    W = 0
+    tf = tb = 0.0
     n = 0
-    kind = "reference" if O.ref_available() else "port"
     for i in order:
-        it = insts[i]
-        w, _ = A.num_column_states(it["trees"], 20)
-        if kind == "reference":
-            # the array form of an ARG implies identity node mappings between
-            # blocks, which only holds before a thread is removed: rebuild the
-            # full ARG and remove the leaf with the reference's own routine
+        w, f, b = _port_time(insts[i], model)
+        W += w
+        tf += f
+        tb += b
+        n += 1
+        if tf + tb >= budget_s:
+            break
+    single = W / (tf + tb)
+    out = dict(value=single, unit="column-states/s", cores=1, kind="port",
+               sample="%d instances drawn W-weighted from the workload (W=%.3g column-states, "
+                      "%.1f s): oracle forward %.1f s + traceback %.1f s, one core"
+               % (n, W, tf + tb, tf, tb))
+    # the compiled reference's forward on the same instances
+    if O.ref_available():
+        times, pops, rho, mu = model
+        fn = O.ref()["arghmm_forward_alg"]
+        fn.restype = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+        Wr, tr_ = 0, 0.0
+        for i in order[:n]:
+            it = insts[i]
             rt = O.ref_new_trees(*it["full"])
             O.ref_remove_thread(rt, it["k"] - 1)
-            O.srand(1)
+            sq = O._Seqs(it["seqs"])
+            dp = ctypes.POINTER(ctypes.c_double)
             t0 = time.perf_counter()
-            O.ref_sample_thread(rt, it["seqs"], times, pops, rho, mu)
-            dt = time.perf_counter() - t0
+            m = fn(ctypes.c_void_p(rt), times.ctypes.data_as(dp), len(times),
+                   pops.ctypes.data_as(dp), ctypes.c_double(rho), ctypes.c_double(mu), sq.ptr,
+                   sq.arr.shape[0], sq.arr.shape[1], ctypes.c_bool(False), None,
+                   ctypes.c_bool(False), ctypes.c_bool(False))
+            tr_ += time.perf_counter() - t0
+            O.ref()["delete_forward_matrix"](m, sq.arr.shape[1])
             O.ref_delete(rt)
-        else:
-            P, Ag, S, bl = A.trees_arrays(it["trees"])
-            maps, seqids = A.trees_internal(it["trees"])
-            inst = O.Instance(times, pops, rho, mu, P, Ag, S, bl, it["seqs"], seqids=seqids,
-                              new_chrom=it["k"] - 1, mappings=maps)
-            t0 = time.perf_counter()
-            fw, _ = inst.forward()
-            inst.traceback(fw, it["u"])
-            dt = time.perf_counter() - t0
-        W += w
-        t_used += dt
-        n += 1
-        if t_used >= budget_s:
-            break
-    return dict(value=W / t_used, unit="column-states/s", cores=1, kind=kind,
-                sample="%d smallest instances of the workload (W=%.3g column-states, %.1f s)%s"
-                % (n, W, t_used, "; arghmm_sample_thread: forward, traceback, recombination "
-                   "sampling and add-thread" if kind == "reference" else
-                   "; oracle forward + traceback"))
+            Wr += it["W"]
+        out["reference_forward_only"] = dict(value=Wr / tr_, seconds=tr_,
+                                             port_forward_only=W / tf)
+    # the host's cores in parallel, each on its own share of the order
+    P = max(1, min(cores, os.cpu_count() or 1))
+    shares = [order[c::P] for c in range(P)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(P) as pool:
+        res = pool.map(_cpu_worker, [(insts, model, sh, budget_s) for sh in shares])
+    wall = time.perf_counter() - t0
+    out["aggregate"] = dict(value=sum(r[0] for r in res) / wall, cores=P, wall_s=wall,
+                            column_states=float(sum(r[0] for r in res)))
+    return out
 
 
 def read_traffic(path):
@@ -145,11 +205,18 @@ def plain_fp64(A, insts, model, budget_s, log):
                 seconds=t_used)
 
 
+def make_serving_pool(A, seed, n=48):
+    """Requests of the config-2 size mix: k ~ U{8..50} haplotypes (one
+    removed), L log-uniform on 5k..100k columns (50 kb .. 1 Mb at c=10),
+    the batched workload's model."""
+    return make_workload(A, n, seed + 1000, kmin=8, kmax=50, lmin=5000, lmax=100000)
+
+
 def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, max_batch=64,
             precision=1):
     """Poisson arrivals at `rate` requests/s of instances drawn from the
-    workload, served with continuous batching (FP32 or FP64); per-request
-    latency = finish - arrival on the server clock."""
+    pool, served with continuous batching (two drivers, FP32 or FP64);
+    per-request latency = finish - arrival on the server clock."""
     import threading
     rng = np.random.RandomState(seed)
     picks = rng.randint(0, len(insts), size=nreq)
@@ -180,14 +247,22 @@ def serving(A, insts, model, rate, nreq, seed, log, budget_gb=96.0, tile=2000, m
     wall = srv.now() - t0
     srv.close()
     lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
-    nb = len(set(r["t_start"] for r in res))
+    nb = len(set((r["driver"], r["t_start"]) for r in res))
+    # requests that ran beside a batch of the other driver
+    spans = [(r["t_start"], r["t_done"], r["driver"]) for r in res]
+    overlapped = sum(1 for a in spans if any(b[2] != a[2] and b[0] < a[1] and a[0] < b[1]
+                                             for b in spans))
     W = float(sum(insts[i]["W"] for i in picks))
     out = dict(rate=rate, requests=nreq, p50_ms=float(np.percentile(lat, 50)),
                p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
                throughput=W / wall, unit="column-states/s", wall_s=wall, batches=nb,
-               max_batch=max_batch, precision="fp32" if precision else "fp64")
-    log("serving %.0f req/s (%s): p50 %.1f ms, p99 %.1f ms, %.3g column-states/s, %d batches"
-        % (rate, out["precision"], out["p50_ms"], out["p99_ms"], out["throughput"], nb))
+               max_batch=max_batch, precision="fp32" if precision else "fp64",
+               overlapped_requests=overlapped,
+               mean_batch=float(np.mean([r["batch_size"] for r in res])))
+    log("serving %.0f req/s (%s): p50 %.1f ms, p99 %.1f ms, %.3g column-states/s, %d batches, "
+        "%d of %d requests beside the other driver's batch"
+        % (rate, out["precision"], out["p50_ms"], out["p99_ms"], out["throughput"], nb,
+           overlapped, nreq))
     return out
 
 
@@ -244,11 +319,34 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
     rng = np.random.RandomState(seed + 21)
     A.set_random_seed(seed + 22)
     steps = []
+    err = None
     for j in range(nsteps):
         leaf = k - 1 - j
         t0 = time.perf_counter()
         A.remove_thread(tr, leaf)
-        r = A.thread_forward(tr, seqs, times, pops, rho, mu, leaf, rng.uniform(size=L),
+        uj = rng.uniform(size=L)
+        if j == 0:
+            # the first step's FP32 forward against FP64 (outside the timing)
+            b = A.Batch()
+            b.add(tr, seqs, times, pops, rho, mu, leaf, uj)
+            b.upload()
+            nst = A.get_nstates(tr, 20, L)
+            b.run(0, 4096, traceback=False)
+            l64, w64 = b.lnl(0), alpha_windows(A, b, 0, L, nst, nwin=8)
+            b.run(1, 4096, traceback=False, fw_tile=50000, fw_warm=20000, fw_tol=1e-4)
+            l32, w32 = b.lnl(0), alpha_windows(A, b, 0, L, nst, nwin=8)
+            ea = er = 0.0
+            for (c0, c1, a64), (_, _, a32) in zip(w64, w32):
+                x, y = window_errors(a64, a32, nst[c0:c1])
+                ea, er = max(ea, x), max(er, y)
+            err = dict(rel_dlogL=abs(l32 - l64) / abs(l64), alpha_max_abs_err_normalised=ea,
+                       alpha_max_rel_err_large_entries=er,
+                       alpha_sample="first step, 8 windows of 256 columns")
+            log("mcmc fp32 step 0 vs fp64: |dlogL|/|logL| %.2e, alpha err abs %.2e rel %.2e"
+                % (err["rel_dlogL"], ea, er))
+            del b
+            t0 = time.perf_counter()
+        r = A.thread_forward(tr, seqs, times, pops, rho, mu, leaf, uj,
                              precision=1, want_alpha=False)
         pos, node, tm = A.sample_recombinations(tr, times, pops, rho, mu, r["path"])
         A.add_thread(tr, len(times), r["path"], leaf, pos, node, tm)
@@ -261,7 +359,8 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
             "trees, log P(seqs | ARG) %.8g, log P(ARG) %.8g"
             % (leaf, dt, r["lnl"], len(pos), steps[-1]["local_trees"],
                steps[-1]["log_likelihood"], steps[-1]["log_prior"]))
-    return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])))
+    return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])),
+                fp32_error=err)
 
 
 def mcmc_cpu_compare(A, seed, log, k=100, L=500000):
@@ -295,6 +394,36 @@ def mcmc_cpu_compare(A, seed, log, k=100, L=500000):
     log("mcmc step vs reference chain: identical ARG %s, local trees %d / %d, "
         "%.2f s GPU, %.2f s CPU" % (same, out["local_trees_gpu"], out["local_trees_cpu"],
                                     t_gpu, t_cpu))
+    return out
+
+
+def window_errors(a64, a32, nst_cols):
+    """FP32 against FP64 on normalised columns: the largest absolute error
+    and the largest relative error on entries above 1e-3 of the column's
+    largest (the bounds stated for the FP32 mode)."""
+    ea = er = 0.0
+    o = 0
+    for n in nst_cols:
+        x, y = a64[o:o + n], a32[o:o + n]
+        o += n
+        if n == 0 or x.sum() <= 0:
+            continue
+        x = x / x.sum()
+        y = y / y.sum()
+        ea = max(ea, float(np.abs(x - y).max()))
+        big = x > 1e-3 * x.max()
+        er = max(er, float((np.abs(x - y)[big] / x[big]).max()))
+    return ea, er
+
+
+def alpha_windows(A, b, i, L, nst, nwin=4, width=256):
+    """Columns of nwin windows spread over the instance (table view)."""
+    out = []
+    for w in range(nwin):
+        c0 = int((w + 0.5) * L / nwin) - width // 2
+        c0 = max(0, min(c0, L - width))
+        c1 = min(L, c0 + width)
+        out.append((c0, c1, b.alpha_range(i, c0, c1, nst)))
     return out
 
 
@@ -362,15 +491,16 @@ def main():
                     help="relative tolerance of a forward tile's start check")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
+                    help="seconds of forward + traceback per core in the CPU baseline")
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="processes of the CPU baseline's aggregate (capped at the host's cores)")
     ap.add_argument("--plain-budget", type=float, default=10.0)
     ap.add_argument("--serve-rates", default="50,200,2000",
                     help="Poisson request rates (req/s), comma-separated")
-    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
     ap.add_argument("--no-large", action="store_true", help="skip the config-2 single instance")
-    ap.add_argument("--serve-fp64-rate", type=float, default=200.0,
-                    help="one FP64 serving run at this rate (0: none)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_fw_all_batched256_traffic.md"),
@@ -383,6 +513,16 @@ def main():
 
     import argweaver_amd
     import argweaver_amd.argweaverc as A
+    t0 = time.time()
+    insts, model = make_workload(A, args.instances, args.seed)
+    log("workload: %d instances generated in %.1f s" % (len(insts), time.time() - t0))
+    # the host-core baseline first: its worker processes are forked before
+    # this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(A, insts, model, args.seed, args.cpu_budget, args.cpu_cores)
+        log("cpu baseline: %.3g column-states/s on one core (%s), %.3g on %d cores"
+            % (cpu["value"], cpu["kind"], cpu["aggregate"]["value"], cpu["aggregate"]["cores"]))
     if not argweaver_amd.device_available():
         raise SystemExit("no HIP device visible")
     if args.calib_copy:
@@ -394,9 +534,6 @@ def main():
         log("calibration copy: 1 GiB read, 1 GiB written")
         del x, y
     log(A.device_info())
-    t0 = time.time()
-    insts, model = make_workload(A, args.instances, args.seed)
-    log("workload: %d instances generated in %.1f s" % (len(insts), time.time() - t0))
 
     batch = A.Batch()
     for it in insts:
@@ -404,7 +541,7 @@ def main():
     batch.upload()
     Wtot = sum(batch.W(i) for i in range(len(insts)))
     for i, it in enumerate(insts):
-        it["W"] = batch.W(i)
+        assert it["W"] == batch.W(i)
     log("uploaded: W = %.4g column-states, columns = %d" % (Wtot, sum(i["L"] for i in insts)))
 
     if args.fw_tile < 0:
@@ -431,7 +568,13 @@ def main():
     def mean(st, k):
         return float(np.mean([x[k] for x in st]))
 
+    # FP32 error against FP64 on a spread of instances (sorted by W, every
+    # 32nd) and windows of their columns
+    err_idx = sorted(range(len(insts)), key=lambda i: insts[i]["W"])[::32]
+    nsts = {i: A.get_nstates(insts[i]["trees"], 20, insts[i]["L"]) for i in err_idx}
+
     wall32, st32 = timed(1)
+    win32 = {i: alpha_windows(A, batch, i, insts[i]["L"], nsts[i]) for i in err_idx}
     log("batched fp32: %.1f ms/step (prep %.1f, emit %.1f, forward %.1f [main launch %.1f, "
         "%d tiles, %d re-run], traceback %.1f, tb passes %d)"
         % (wall32 * 1e3, mean(st32, "ms_prep"), mean(st32, "ms_emit"), mean(st32, "ms_forward"),
@@ -447,6 +590,19 @@ def main():
         wall64, st64 = timed(0)
         lnl64 = [batch.lnl(i) for i in range(len(insts))]
         rel = max(abs(a - b) / abs(b) for a, b in zip(lnl32, lnl64))
+        ea = er = 0.0
+        for i in err_idx:
+            for (c0, c1, a32), (_, _, a64) in zip(win32[i], alpha_windows(A, batch, i, insts[i]["L"],
+                                                                         nsts[i])):
+                x, y = window_errors(a64, a32, nsts[i][c0:c1])
+                ea, er = max(ea, x), max(er, y)
+        extra["fp32_error"] = dict(
+            max_rel_dlogL=rel, alpha_max_abs_err_normalised=ea,
+            alpha_max_rel_err_large_entries=er,
+            alpha_sample="%d instances (every 32nd by W), 4 windows of 256 columns each"
+            % len(err_idx))
+        log("batched fp32 vs fp64: max |dlogL|/|logL| %.2e, alpha err abs %.2e rel %.2e"
+            % (rel, ea, er))
         log("batched fp64: %.1f ms/step; max |dlogL|/|logL| fp32 vs fp64 %.2e"
             % (wall64 * 1e3, rel))
         plain = plain_fp64(A, insts, model, args.plain_budget, log)
@@ -458,13 +614,11 @@ def main():
         extra["max_rel_dlogL_fp32_vs_fp64"] = rel
 
     if args.serve_rates:
-        extra["serving"] = [serving(A, insts, model, float(r), args.serve_requests, args.seed + j,
-                                    log, tile=args.tile)
+        pool, pmodel = make_serving_pool(A, args.seed)
+        extra["serving"] = [serving(A, pool, pmodel, float(r), args.serve_requests,
+                                    args.seed + j, log, tile=args.tile, precision=prec)
+                            for prec in (1, 0)
                             for j, r in enumerate(args.serve_rates.split(","))]
-        if args.serve_fp64_rate > 0:
-            extra["serving"].append(serving(A, insts, model, args.serve_fp64_rate,
-                                            args.serve_requests, args.seed + 97, log,
-                                            tile=args.tile, precision=0))
 
     if not args.no_large:
         extra["single_large"] = single_large(A, args.seed, log)
@@ -487,10 +641,7 @@ def main():
                     kernel="k_fw_all<float, 19, false>", algorithmic_bytes_per_launch=Wtot * 4,
                     ms_per_launch=fwd_ms)
 
-    cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(A, insts, model, args.cpu_budget)
-        log("cpu baseline: %.3g column-states/s (%s)" % (cpu["value"], cpu["kind"]))
         if not args.no_mcmc:
             cmp = mcmc_cpu_compare(A, args.seed, log)
             if cmp is not None:

@@ -381,6 +381,9 @@ int awamd_batch_tune_forward(void *batch, const int *tiles, const int *warms, in
                              double tol, double *ms_out);
 int awamd_batch_path(void *batch, int i, int *path);
 int awamd_batch_alpha(void *batch, int i, double *alpha);
+/* columns [c0, c1) of instance i's table, as awamd_batch_alpha lays them out
+ * (the states of each column's block in state order); 1 for a bad range */
+int awamd_batch_alpha_range(void *batch, int i, int c0, int c1, double *alpha);
 int awamd_batch_norms(void *batch, int i, double *norms);
 double awamd_batch_lnl(void *batch, int i);
 long long awamd_batch_W(void *batch, int i);
