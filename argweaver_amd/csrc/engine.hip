@@ -49,6 +49,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -1940,7 +1941,8 @@ std::recursive_mutex &device_mutex()
 #define DEVICE_LOCK std::lock_guard<std::recursive_mutex> device_lock_(device_mutex())
 
 // The batch's streams: main carries every stage; side carries only the
-// traceback tiles of the instances whose forward is final while the forward
+// traceback of the instances whose forward is final (their options,
+// re-runs and adoptions, driven by a helper host thread) while the forward
 // re-runs of others continue on main.  The two are joined by the host
 // (stream synchronisation), never by a stream waiting on an event.
 struct Streams {
@@ -2362,11 +2364,11 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
         m.d_chunks = p.take<Chunk>(nch);
         m.d_ostart = p.take<int>((size_t)nch * K);
         m.d_oend = p.take<int>((size_t)nch * K);
-        m.d_ridx = p.take<int>(nch);
-        m.d_rk1 = p.take<int>(nch);
-        m.d_rres = p.take<int>(3 * (size_t)nch);
-        m.d_aidx = p.take<int>(nch);
-        m.d_aopt = p.take<int>(nch);
+        m.d_ridx = p.take<int>(2 * (size_t)nch);
+        m.d_rk1 = p.take<int>(2 * (size_t)nch);
+        m.d_rres = p.take<int>(6 * (size_t)nch);
+        m.d_aidx = p.take<int>(2 * (size_t)nch);
+        m.d_aopt = p.take<int>(2 * (size_t)nch);
         m.d_tlist = p.take<int>(2 * (size_t)nch);
         m.d_obuf = p.take<int>((size_t)std::max(1LL, (long long)(K - 1) * ocols));
         if (pass == 0 && m.tiles.ensure(p.used + 4096)) return 1;
@@ -2411,11 +2413,17 @@ static int launch_options(Batch::Impl &m, bool fp32, size_t sht, const std::vect
 // prediction.  A re-run replaces option 0 by the chain from its start (the
 // samples until it coalesces with a stored option, then that option), so a
 // right prediction leaves the tile decided for the next pass.
-static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
+// take: the instances to resolve (all when null); slot 0 or 1 of the
+// device lists, so that two resolutions of disjoint instances can run at
+// once on two streams.
+static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs, hipStream_t s,
+                         const std::vector<char> *take, int slot)
 {
-    hipStream_t s = m.st.main;
     const std::vector<Chunk> &chs = m.chunks;
     const int nch = (int)chs.size(), K = m.tb_opts;
+    int *ridx = m.d_ridx + (size_t)slot * nch, *rk1 = m.d_rk1 + (size_t)slot * nch;
+    int *rres = m.d_rres + (size_t)slot * 3 * nch;
+    int *aidx = m.d_aidx + (size_t)slot * nch, *aopt = m.d_aopt + (size_t)slot * nch;
     std::vector<int> os((size_t)nch * K), oe((size_t)nch * K);
     HIP_OK(hipMemcpyAsync(os.data(), m.d_ostart, sizeof(int) * os.size(), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(oe.data(), m.d_oend, sizeof(int) * oe.size(), hipMemcpyDeviceToHost, s));
@@ -2432,7 +2440,7 @@ static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
     };
     std::vector<Walk> ws;
     for (int j = 0; j < nch; j++)
-        if (chs[j].last && !chs[j].first) {
+        if (chs[j].last && !chs[j].first && (!take || (*take)[chs[j].inst])) {
             int f = j;
             while (!chs[f].first) f--;
             ws.push_back(Walk{f, j - 1, oe[(size_t)j * K]});
@@ -2469,17 +2477,17 @@ static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
         }
         if (pj.empty()) break;
         const int np = (int)pj.size();
-        HIP_OK(hipMemcpyAsync(m.d_ridx, pj.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
-        HIP_OK(hipMemcpyAsync(m.d_rk1, pk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(ridx, pj.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(rk1, pk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
         if (fp32 ? launch_lds(k_tb_rerun<float>, np, sht, s, (const DevInst *)m.dD,
-                              (const Chunk *)m.d_chunks, (const int *)m.d_ridx,
-                              (const int *)m.d_rk1, K, m.d_obuf, (const int *)m.d_ostart, m.d_rres)
+                              (const Chunk *)m.d_chunks, (const int *)ridx,
+                              (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres)
                  : launch_lds(k_tb_rerun<double>, np, sht, s, (const DevInst *)m.dD,
-                              (const Chunk *)m.d_chunks, (const int *)m.d_ridx,
-                              (const int *)m.d_rk1, K, m.d_obuf, (const int *)m.d_ostart, m.d_rres))
+                              (const Chunk *)m.d_chunks, (const int *)ridx,
+                              (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres))
             return 1;
         res.resize(3 * (size_t)np);
-        HIP_OK(hipMemcpyAsync(res.data(), m.d_rres, sizeof(int) * res.size(), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(res.data(), rres, sizeof(int) * res.size(), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         rs.tb_passes++;
         for (int x = 0; x < np; x++) {
@@ -2491,10 +2499,10 @@ static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs)
     }
     if (!aj.empty()) {
         const int na = (int)aj.size();
-        HIP_OK(hipMemcpyAsync(m.d_aidx, aj.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
-        HIP_OK(hipMemcpyAsync(m.d_aopt, ao.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(aidx, aj.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(aopt, ao.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_tb_adopt, dim3(na), dim3(256), 0, s, (const DevInst *)m.dD,
-                           (const Chunk *)m.d_chunks, (const int *)m.d_aidx, (const int *)m.d_aopt,
+                           (const Chunk *)m.d_chunks, (const int *)aidx, (const int *)aopt,
                            m.d_obuf);
         HIP_OK(hipGetLastError());
     }
@@ -2512,7 +2520,7 @@ static int traceback_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt, R
     const int K = std::max(1, std::min(opt.tile_opts, TB_KMAX));
     if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
     if (launch_options(m, fp32, sht, nullptr, 0, m.st.main)) return 1;
-    return opt.fixup ? resolve_tiles(m, fp32, sht, rs) : 0;
+    return opt.fixup ? resolve_tiles(m, fp32, sht, rs, m.st.main, nullptr, 0) : 0;
 }
 
 int Batch::tune_tiles(Precision prec, const int *cand, int ncand, double *ms_out, int warm,
@@ -2894,18 +2902,35 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     // Once the first check of the forward tiles has shown which instances
     // are re-run, the traceback options of all the others go to the side
     // stream and run while the re-runs continue on the main stream.
+    // A helper host thread then resolves those instances' tiles on the side
+    // stream (its re-runs too), while this thread drives the forward re-runs
+    // and then the re-run instances' traceback on the main stream.
     size_t sht = 0;
-    std::vector<char> busy;
+    std::vector<char> busy, done;
     const int K = std::max(1, std::min(opt.tile_opts, TB_KMAX));
+    std::thread helper;
+    RunStats rs_side;
+    int rc_side = 0;
     auto early_tb = [&](const std::vector<char> &b) -> int {
         if (!opt.traceback || std::find(b.begin(), b.end(), 1) == b.end()) return 0;
         busy = b;
         if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
         HIP_OK(hipStreamSynchronize(s));   // the tile list is on the device
-        std::vector<char> done(n);
+        done.assign(n, 0);
         for (int i = 0; i < n; i++) done[i] = !b[i];
-        return launch_options(m, fp32, sht, &done, 0, c.side);
+        if (launch_options(m, fp32, sht, &done, 0, c.side)) return 1;
+        if (opt.fixup)
+            helper = std::thread([&] { rc_side = resolve_tiles(m, fp32, sht, rs_side, c.side, &done, 1); });
+        return 0;
     };
+    // the helper is joined on every path out of run()
+    struct Joiner {
+        std::thread &t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } joiner{helper};
     if (forward_stage(m, fp32, opt, order, shf, t1max, rs, early_tb)) return 1;
     (void)hipEventRecord(c.ev[4], s);
     hipLaunchKernelGGL(k_lnl_b, dim3(n), dim3(256), 0, s, m.dD, m.d_list);
@@ -2916,8 +2941,13 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
             if (traceback_stage(m, fp32, opt, rs)) return 1;
         } else {
             if (launch_options(m, fp32, sht, &busy, 1, s)) return 1;
+            if (opt.fixup && resolve_tiles(m, fp32, sht, rs, s, &busy, 0)) return 1;
+            if (helper.joinable()) helper.join();
             HIP_OK(hipStreamSynchronize(c.side));
-            if (opt.fixup && resolve_tiles(m, fp32, sht, rs)) return 1;
+            if (rc_side) return 1;
+            rs.tb_passes = std::max(rs.tb_passes, rs_side.tb_passes);
+            rs.tb_rerun_cols += rs_side.tb_rerun_cols;
+            rs.tb_adopted += rs_side.tb_adopted;
         }
     }
     (void)hipEventRecord(c.ev[5], s);
