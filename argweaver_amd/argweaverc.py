@@ -86,6 +86,14 @@ argweaver_sample_arg_refine = _export(
     "arghmm_sample_arg_refine", c_void_p,
     [c_double_p, c_int, c_double_p, c_double, c_double,
      c_char_p_p, c_int, c_int, c_int, c_int])
+argweaver_resample_arg_region = _export(
+    "arghmm_resample_arg_region", c_void_p,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, c_int, c_int, c_int])
+argweaver_resample_mcmc_arg = _export(
+    "arghmm_resample_mcmc_arg", c_void_p,
+    [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
+     c_char_p_p, c_int, c_int, c_int, c_int, c_int])
 argweaver_resample_arg = _export(
     "arghmm_resample_arg", c_void_p,
     [c_void_p, c_double_p, c_int, c_double_p, c_double, c_double,
@@ -391,6 +399,34 @@ def resample_arg(trees, seqs, times, popsizes, rho, mu, niters=1, nremove=1):
     cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
     return argweaver_resample_arg(trees, _dptr(times), len(times), _dptr(popsizes),
                                   rho, mu, cs.ptr, cs.nseqs, cs.seqlen, niters, nremove)
+
+
+def resample_arg_region(trees, seqs, times, popsizes, rho, mu, region_start, region_end,
+                        niters=1):
+    """arghmm_resample_arg_region: conditioned internal re-threadings of the
+    columns [region_start, region_end) (MCMC accept / reject)."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    out = argweaver_resample_arg_region(trees, _dptr(times), len(times), _dptr(popsizes), rho, mu,
+                                        cs.ptr, cs.nseqs, cs.seqlen, region_start, region_end,
+                                        niters)
+    if not out:
+        raise RuntimeError("arghmm_resample_arg_region failed")
+    return out
+
+
+def resample_mcmc_arg(trees, seqs, times, popsizes, rho, mu, niters=1, niters2=1, window=200):
+    """arghmm_resample_mcmc_arg: leaf re-threadings and windowed region
+    re-threadings, chosen at random per iteration."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    out = argweaver_resample_mcmc_arg(trees, _dptr(times), len(times), _dptr(popsizes), rho, mu,
+                                      cs.ptr, cs.nseqs, cs.seqlen, niters, niters2, window)
+    if not out:
+        raise RuntimeError("arghmm_resample_mcmc_arg failed")
+    return out
 
 
 def sample_arg_seq(seqs, times, popsizes, rho, mu):

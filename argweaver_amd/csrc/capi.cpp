@@ -19,6 +19,7 @@
 #include "local_trees.h"
 #include "thread_path.h"
 #include "internal_prep.h"
+#include "region.h"
 
 namespace aw {
 
@@ -179,9 +180,11 @@ static int sample_arg_thread(ArgTrees &tr, const double *times, int ntimes, cons
 // The internal-branch instance of an ARG whose trees are in
 // subtree/maintree form: per-block inputs prepared on the host
 // (prepare_internal), forward and traceback on the device.
+// start_state >= 0: the first column is 1 on that internal state of the
+// first block (a conditioned start), else the internal prior
 static int build_internal(const ArgTrees &tr, const double *times, int ntimes,
                           const double *popsizes, double rho, double mu, char **seqs, int minage,
-                          HostInstance &hi)
+                          HostInstance &hi, int start_state = -1)
 {
     if (ntimes < 2 || ntimes > MAXT) {
         fprintf(stderr, "argweaver_amd: %d time points (supported: 2..%d)\n", ntimes, MAXT);
@@ -194,6 +197,11 @@ static int build_internal(const ArgTrees &tr, const double *times, int ntimes,
     model_init(hi.model, ntimes, times, pop.data(), rho, mu);
     auto hp = std::make_shared<HostPrep>();
     if (prepare_internal(tr, hi.model, seqs, minage, *hp)) return 1;
+    if (start_state >= 0) {
+        // the first column given: 1 on the start state
+        hp->prior.assign(hp->prior.size(), 0.0);
+        hp->prior[lane_pos(start_state, hp->R)] = 1.0;
+    }
     hi.prep = hp;
     return 0;
 }
@@ -225,6 +233,136 @@ static int sample_arg_thread_internal(ArgTrees &tr, const double *times, int nti
     std::vector<int> rpos, rnode, rtime;
     sample_recombinations_internal(tr, hi.model, minage, path.data(), rpos, rnode, rtime);
     return add_thread_path(tr, ntimes, minage, path.data(), rpos, rnode, rtime);
+}
+
+// index of state (node, time) among the internal states of tree t (-1 when
+// absent)
+static int internal_state_index(const LTree &t, int ntimes, int node, int time)
+{
+    std::vector<int> sn, st;
+    internal_states(t, ntimes, 0, sn, st);
+    for (size_t k = 0; k < sn.size(); k++)
+        if (sn[k] == node && st[k] == time) return (int)k;
+    return -1;
+}
+
+// cond_sample_arg_thread_internal (sample_thread.cpp:771-865): re-thread the
+// removed subtree with the first column's state given (start_node != -1) and
+// the last column's state given (end_node != -1); an end without states (a
+// complete tree) is given as its placeholder.  The last column's uniform is
+// drawn only when its state is sampled.
+static int cond_sample_arg_thread_internal(ArgTrees &tr, const double *times, int ntimes,
+                                           const double *popsizes, double rho, double mu,
+                                           char **seqs, int start_node, int start_time,
+                                           int end_node, int end_time)
+{
+    const LTree &first = tr.blocks.front().tree, &last = tr.blocks.back().tree;
+    int j = -1;
+    if (start_node != -1 && first.nodes[first.root].age >= ntimes) {
+        j = internal_state_index(first, ntimes, start_node, start_time);
+        if (j < 0) {
+            fprintf(stderr, "argweaver_amd: conditioned start state not in the first tree\n");
+            return 1;
+        }
+    }
+    HostInstance hi;
+    if (build_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0, hi, j)) return 1;
+    int k = -1;
+    if (last.nodes[last.root].age < ntimes) {
+        k = 0;   // a complete tree: its placeholder state
+    } else if (end_node != -1) {
+        k = internal_state_index(last, ntimes, end_node, end_time);
+        if (k < 0) {
+            fprintf(stderr, "argweaver_amd: conditioned end state not in the last tree\n");
+            return 1;
+        }
+    }
+    hi.last_state = k;
+    const int L = tr.length();
+    std::vector<double> u(L, 0.0);
+    for (int i = (k >= 0) ? 1 : 0; i < L; i++) u[i] = frand();
+    std::vector<int> path(L, 0);
+    if (run_instance(hi, u.data(), path.data(), nullptr, nullptr, nullptr)) return 1;
+    std::vector<int> rpos, rnode, rtime;
+    sample_recombinations_internal(tr, hi.model, 0, path.data(), rpos, rnode, rtime);
+    return add_thread_path(tr, ntimes, 0, path.data(), rpos, rnode, rtime);
+}
+
+// resample_arg_region (sample_arg.cpp:429-527): niters re-threadings of a
+// uniformly drawn branch over the region [rstart, rend), each conditioned on
+// the states the branch held at the region's ends (open at an end of the
+// ARG) and accepted with probability exp(npaths - npaths2); the region is
+// then spliced back.  Returns the acceptance rate, -1 on failure.
+static double resample_arg_region(ArgTrees &tr, const double *times, int ntimes,
+                                  const double *popsizes, double rho, double mu, char **seqs,
+                                  int rstart, int rend, int niters, bool open_ended)
+{
+    const int maxtime = ntimes + 1;
+    if (rstart == rend) return 1.0;
+    if (rstart < tr.start_coord || rend > tr.end_coord || rstart > rend) {
+        fprintf(stderr, "argweaver_amd: region [%d, %d) outside the ARG [%d, %d)\n", rstart,
+                rend, tr.start_coord, tr.end_coord);
+        return -1;
+    }
+    const int arg_start = tr.start_coord;
+    ArgTrees t2 = partition_trees(tr, rstart);
+    ArgTrees t3 = partition_trees(t2, rend);
+    // a zero-length last block is extended by one column while the region
+    // is re-threaded
+    const bool stub = t2.blocks.back().blocklen == 0;
+    if (stub) {
+        t2.blocks.back().blocklen++;
+        t2.end_coord++;
+    }
+    int accepts = 0;
+    for (int it = 0; it < niters; it++) {
+        ArgTrees old = t2;
+        const LTree start_tree = t2.blocks.front().tree, end_tree = t2.blocks.back().tree;
+        std::vector<int> path(t2.blocks.size());
+        const double npaths = removal_path_uniform(t2, path.data());
+        remove_thread_path(t2, path.data(), maxtime, nullptr);
+        int sn, st, en, et;
+        find_state_sub_tree_internal(start_tree, t2.blocks.front().tree, maxtime, sn, st);
+        find_state_sub_tree_internal(end_tree, t2.blocks.back().tree, maxtime, en, et);
+        if (open_ended) {
+            if (rstart == arg_start) sn = st = -1;
+            if (rend == t3.end_coord) en = et = -1;
+        }
+        if (cond_sample_arg_thread_internal(t2, times, ntimes, popsizes, rho, mu, seqs, sn, st,
+                                            en, et))
+            return -1;
+        const double npaths2 = count_removal_paths(t2);
+        const bool accept = frand() < exp(npaths - npaths2);
+        if (!accept) t2 = std::move(old);
+        else accepts++;
+    }
+    if (stub) {
+        t2.blocks.back().blocklen--;
+        t2.end_coord--;
+    }
+    append_trees(tr, t2);
+    append_trees(tr, t3);
+    return accepts / double(niters);
+}
+
+// resample_arg_regions (sample_arg.cpp:532-552): windows of `window`
+// columns every `step` columns
+static double resample_arg_regions(ArgTrees &tr, const double *times, int ntimes,
+                                   const double *popsizes, double rho, double mu, char **seqs,
+                                   int window, int step, int niters)
+{
+    double rate = 0.0;
+    int nwindows = 0;
+    for (int start = tr.start_coord; start == tr.start_coord || start + window / 2 < tr.end_coord;
+         start += step) {
+        nwindows++;
+        const int end = std::min(start + window, tr.end_coord);
+        const double r = resample_arg_region(tr, times, ntimes, popsizes, rho, mu, seqs, start, end,
+                                             niters, true);
+        if (r < 0) return -1;
+        rate += r;
+    }
+    return rate / nwindows;
 }
 
 static bool require_gpu(const char *fn)
@@ -626,6 +764,46 @@ static int resample_arg_leaf(ArgTrees &tr, const double *times, int ntimes,
     removal_leaf_path(tr, node, path.data());
     remove_thread_path(tr, path.data(), ntimes + 1, nullptr);
     return sample_arg_thread_internal(tr, times, ntimes, popsizes, rho, mu, seqs, 0);
+}
+
+LocalTrees *arghmm_resample_arg_region(LocalTrees *trees, double *times, int ntimes,
+                                       double *popsizes, double rho, double mu, char **seqs,
+                                       int nseqs, int seqlen, int region_start, int region_end,
+                                       int niters)
+{
+    (void)nseqs;
+    (void)seqlen;
+    if (!require_gpu("arghmm_resample_arg_region")) return nullptr;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    if (resample_arg_region(tr, times, ntimes, popsizes, rho, mu, seqs, region_start, region_end,
+                            niters, true) < 0)
+        return nullptr;
+    return trees;
+}
+
+// resample_arg_mcmc_all (sample_arg.cpp:190-203) per iteration: a leaf's
+// re-threading with probability frac_leaf = 0.5, else windows of `window`
+// columns every window / 2 columns, niters2 re-threadings each
+LocalTrees *arghmm_resample_mcmc_arg(LocalTrees *trees, double *times, int ntimes,
+                                     double *popsizes, double rho, double mu, char **seqs,
+                                     int nseqs, int seqlen, int niters, int niters2, int window)
+{
+    (void)seqlen;
+    if (!require_gpu("arghmm_resample_mcmc_arg")) return nullptr;
+    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    const double frac_leaf = 0.5;
+    const int step = window / 2;
+    for (int c = tr.num_leaves(); c < nseqs; c++)
+        if (sample_arg_thread(tr, times, ntimes, popsizes, rho, mu, seqs, c)) return nullptr;
+    for (int i = 0; i < niters; i++) {
+        if (frand() < frac_leaf) {
+            if (resample_arg_leaf(tr, times, ntimes, popsizes, rho, mu, seqs)) return nullptr;
+        } else if (resample_arg_regions(tr, times, ntimes, popsizes, rho, mu, seqs, window, step,
+                                        niters2) < 0) {
+            return nullptr;
+        }
+    }
+    return trees;
 }
 
 LocalTrees *arghmm_resample_all_arg(LocalTrees *trees, double *times, int ntimes,

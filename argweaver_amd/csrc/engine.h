@@ -51,6 +51,7 @@ struct DevInst {
     double *lcoef;            // [B*NCOEF*RW] coefficient planes (see Coef), FP64 runs
     float *lcoef32;           // [B*NCOEF*RW] the planes in FP32, FP32 runs
     int fp32;                 // the run is FP32
+    int last_state;           // the state of the last column when given, else -1 (sampled)
     int hostprep;             // the per-block inputs come from the host (internal threading)
     int *ldet;                // [B*RW] deterministic switch target of block b-1's states
     double *lprior;           // [RW] first-column prior
@@ -118,6 +119,10 @@ struct HostInstance {
     // internal threading: the per-block inputs prepared on the host (the
     // device preparation, classification and emission kernels skip it)
     std::shared_ptr<const HostPrep> prep;
+    // the last column's state when the traceback is conditioned on it (its
+    // index in the last block's state order), else -1: the last column's
+    // uniform (u[0]) is then not used
+    int last_state = -1;
 };
 
 // Schedule of a batch run.  The forward runs one wave per instance over

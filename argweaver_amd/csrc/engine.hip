@@ -1745,7 +1745,14 @@ __device__ __forceinline__ void tb_option(const DevInst d, const Chunk &ch, int 
             if (threadIdx.x == 0) *ostart = *oend = -1;
             return;
         }
-        c = tb_column<Real, R>(d, L - 1, cp(d.u)[0]);
+        if (d.last_state >= 0) {
+            // conditioned on the last column's state
+            c.blk = d.B - 1;
+            c.k = d.last_state;
+            c.info = gp(d.linfo)[(size_t)c.blk * RW + lane_pos(c.k, R)];
+        } else {
+            c = tb_column<Real, R>(d, L - 1, cp(d.u)[0]);
+        }
         if (threadIdx.x == 0) path[L - 1] = c.k;
         tb_range<Real, R>(d, tab, rows, L - 2, ch.c0, c, path, L - 2, none, dummy);
         if (threadIdx.x == 0) {
@@ -2264,6 +2271,7 @@ int Batch::upload()
                     return 1;
                 }
                 d.hostprep = hi.prep ? 1 : 0;
+                d.last_state = hi.last_state;
                 d.T = T; d.N = N; d.B = B; d.L = L; d.nrows = (int)hi.rows.size();
                 d.Smax = Smax; d.TC = tc_base(T - 1, T - 1); d.W = m.W[i]; d.R = R;
             }

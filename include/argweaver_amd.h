@@ -116,6 +116,22 @@ LocalTrees *arghmm_resample_all_arg(LocalTrees *trees, double *times, int ntimes
 LocalTrees *arghmm_resample_climb_arg(LocalTrees *trees, double *times, int ntimes,
                                       double *popsizes, double rho, double mu, char **seqs,
                                       int nseqs, int seqlen, int niters, double recomb_preference);
+/* src/argweaver/sample_arg.cpp:828-842 (resample_arg_region, :429-527):
+ * niters re-threadings of a uniformly drawn branch over the columns
+ * [region_start, region_end), each conditioned on the states the branch
+ * held at the region's ends (open at an end of the ARG) and accepted with
+ * probability exp(npaths - npaths2); NULL on failure */
+LocalTrees *arghmm_resample_arg_region(LocalTrees *trees, double *times, int ntimes,
+                                       double *popsizes, double rho, double mu, char **seqs,
+                                       int nseqs, int seqlen, int region_start, int region_end,
+                                       int niters);
+/* src/argweaver/sample_arg.cpp:743-765 (resample_arg_mcmc_all, :190-203):
+ * per iteration a leaf's re-threading (probability 1/2) or the region
+ * re-threading over windows of `window` columns every window / 2, niters2
+ * re-threadings each */
+LocalTrees *arghmm_resample_mcmc_arg(LocalTrees *trees, double *times, int ntimes,
+                                     double *popsizes, double rho, double mu, char **seqs,
+                                     int nseqs, int seqlen, int niters, int niters2, int window);
 /* src/argweaver/sample_thread.cpp:1026-1029 */
 void delete_path(int *path);
 /* src/argweaver/logging.cpp:126-127 */

@@ -7,7 +7,10 @@ same removal path and the same rand() stream:
   form, trees left complete by the path (one placeholder state) included;
 - arghmm_sample_arg_thread_internal (:981-1006): the identical thread;
 - arghmm_resample_all_arg and arghmm_resample_climb_arg
-  (sample_arg.cpp:722-806): the identical ARG after several re-threadings.
+  (sample_arg.cpp:722-806): the identical ARG after several re-threadings;
+- arghmm_resample_arg_region and arghmm_resample_mcmc_arg (conditioned
+  re-threading of a region, accept / reject, cutting and joining the ARG):
+  the identical ARG.
 """
 import ctypes
 from ctypes import POINTER, c_bool, c_double, c_int, c_void_p
@@ -138,3 +141,93 @@ def test_internal_resampling_matches_reference(aw, name, extra, k, L, seed):
     assert _same(aw, tr, rt)
     O.ref_delete(rt)
     aw.delete_local_trees(tr)
+
+
+def _block_starts(aw, tr):
+    bl = aw.trees_arrays(tr)[3]
+    return np.concatenate([[0], np.cumsum(bl)])[:-1]
+
+
+@pytest.mark.parametrize("k,L,seed,where", [(6, 3000, 31, "middle"), (8, 4000, 32, "start"),
+                                            (8, 4000, 33, "end"), (10, 5000, 34, "block")])
+def test_resample_arg_region_matches_reference(aw, k, L, seed, where):
+    """arghmm_resample_arg_region (sample_arg.cpp:429-527, 828-842): the ARG
+    cut at the region's ends (a zero-length block when the start is a block
+    start), three conditioned internal re-threadings with accept / reject,
+    and the pieces joined again (congruent mapping or SPR repair) give the
+    reference's ARG exactly."""
+    print("region k=%d L=%d seed=%d %s" % (k, L, seed, where), flush=True)
+    tr, rt, seqs, (times, pops, rho, mu) = _pair(aw, k, L, seed)
+    if where == "middle":
+        r0, r1 = L // 3, 2 * L // 3
+    elif where == "start":
+        r0, r1 = 0, L // 2
+    elif where == "end":
+        r0, r1 = L // 2, L
+    else:
+        starts = _block_starts(aw, tr)
+        r0 = int(starts[len(starts) // 3])
+        r1 = int(starts[2 * len(starts) // 3])
+        assert 0 < r0 < r1 < L
+    s = O._Seqs(seqs)
+    args = (_d(times), c_int(len(times)), _d(pops), c_double(rho), c_double(mu), s.ptr,
+            c_int(s.arr.shape[0]), c_int(L), c_int(r0), c_int(r1), c_int(3))
+    fm = aw._lib["arghmm_resample_arg_region"]
+    fm.restype = c_void_p
+    fr = O.ref()["arghmm_resample_arg_region"]
+    fr.restype = c_void_p
+    O.srand(500 + seed)
+    assert fm(c_void_p(tr), *args)
+    O.srand(500 + seed)
+    fr(c_void_p(rt), *args)
+    assert _same(aw, tr, rt)
+    O.ref_delete(rt)
+    aw.delete_local_trees(tr)
+
+
+@pytest.mark.parametrize("k,L,seed,window", [(6, 3000, 41, 1000), (8, 4000, 42, 1500)])
+def test_resample_mcmc_arg_matches_reference(aw, k, L, seed, window):
+    """arghmm_resample_mcmc_arg (sample_arg.cpp:190-203, 532-552, 743-765):
+    leaf re-threadings and windowed region re-threadings chosen at random
+    give the reference's ARG exactly."""
+    print("mcmc k=%d L=%d seed=%d window=%d" % (k, L, seed, window), flush=True)
+    tr, rt, seqs, (times, pops, rho, mu) = _pair(aw, k, L, seed)
+    s = O._Seqs(seqs)
+    args = (_d(times), c_int(len(times)), _d(pops), c_double(rho), c_double(mu), s.ptr,
+            c_int(s.arr.shape[0]), c_int(L), c_int(3), c_int(2), c_int(window))
+    fm = aw._lib["arghmm_resample_mcmc_arg"]
+    fm.restype = c_void_p
+    fr = O.ref()["arghmm_resample_mcmc_arg"]
+    fr.restype = c_void_p
+    O.srand(600 + seed)
+    assert fm(c_void_p(tr), *args)
+    O.srand(600 + seed)
+    fr(c_void_p(rt), *args)
+    assert _same(aw, tr, rt)
+    O.ref_delete(rt)
+    aw.delete_local_trees(tr)
+
+
+def test_resample_arg_region_changes_the_arg(aw):
+    """The region re-threadings above are not all rejected: with more
+    iterations the ARG changes (and still equals the reference's)."""
+    changed = 0
+    for seed in (51, 52, 53):
+        tr, rt, seqs, (times, pops, rho, mu) = _pair(aw, 8, 4000, seed)
+        before = aw.trees_arrays(tr)
+        s = O._Seqs(seqs)
+        args = (_d(times), c_int(len(times)), _d(pops), c_double(rho), c_double(mu), s.ptr,
+                c_int(s.arr.shape[0]), c_int(4000), c_int(1000), c_int(3000), c_int(6))
+        fm = aw._lib["arghmm_resample_arg_region"]
+        fm.restype = c_void_p
+        fr = O.ref()["arghmm_resample_arg_region"]
+        fr.restype = c_void_p
+        O.srand(700 + seed)
+        assert fm(c_void_p(tr), *args)
+        O.srand(700 + seed)
+        fr(c_void_p(rt), *args)
+        assert _same(aw, tr, rt)
+        changed += not all(np.array_equal(a, b) for a, b in zip(before, aw.trees_arrays(tr)))
+        O.ref_delete(rt)
+        aw.delete_local_trees(tr)
+    assert changed >= 1
