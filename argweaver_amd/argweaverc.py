@@ -207,6 +207,7 @@ awamd_batch_run_ex = _export(
     "awamd_batch_run_ex", c_int,
     [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p])
 awamd_batch_set_traceback = _export("awamd_batch_set_traceback", c_int, [c_void_p, c_int, c_int])
+awamd_batch_set_tail_tiles = _export("awamd_batch_set_tail_tiles", c_int, [c_void_p, c_int])
 awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
 awamd_batch_tune_forward = _export(
     "awamd_batch_tune_forward", c_int, [c_void_p, c_int_p, c_int_p, c_int, c_double, c_double_p])
@@ -829,7 +830,8 @@ class Batch(object):
     """Many independent threading instances resident in HBM (awamd_batch_*)."""
 
     STAT_NAMES = ("ms_prep", "ms_emit", "ms_forward", "ms_traceback", "ms_total", "W",
-                  "nvar", "tb_passes", "tb_rerun_cols", "fw_tiles", "fw_reruns", "ms_fw_main")
+                  "nvar", "tb_passes", "tb_rerun_cols", "fw_tiles", "fw_reruns", "ms_fw_main",
+                  "ms_tb_tail", "ms_tb_side")
 
     def __init__(self):
         self.ptr = awamd_batch_new()
@@ -871,6 +873,13 @@ class Batch(object):
         on them."""
         if awamd_batch_set_traceback(self.ptr, int(warm), int(options)) != 0:
             raise ValueError("traceback warm-up / options out of range")
+
+    def set_tail_tiles(self, cols):
+        """Traceback tile length of the instances whose FP32 forward tiles
+        are re-run (0: the run's tile length); the thread does not depend on
+        it."""
+        if awamd_batch_set_tail_tiles(self.ptr, int(cols)) != 0:
+            raise ValueError("tail traceback tile length out of range")
 
     def tune_forward(self, candidates=((0, 0), (60000, 25000), (80000, 25000), (90000, 35000),
                                        (100000, 40000), (110000, 45000), (120000, 50000),

@@ -939,6 +939,7 @@ struct AwBatch {
     std::vector<HostInstance> staged;
     int tb_warm = BatchOptions().tile_warm;
     int tb_opts = BatchOptions().tile_opts;
+    int tb_tail = BatchOptions().tail_tile_cols;
 };
 
 void *awamd_batch_new(void) { return new AwBatch(); }
@@ -975,6 +976,7 @@ int awamd_batch_run_opts(void *b, int precision, int tile_cols, int traceback, i
     opt.tile_cols = tile_cols;
     opt.tile_warm = ab.tb_warm;
     opt.tile_opts = ab.tb_opts;
+    opt.tail_tile_cols = ab.tb_tail;
     opt.traceback = traceback != 0;
     opt.fixup = fixup != 0;
     RunStats st;
@@ -1006,6 +1008,16 @@ int awamd_batch_set_traceback(void *b, int warm, int options)
     return 0;
 }
 
+int awamd_batch_set_tail_tiles(void *b, int cols)
+{
+    if (cols < 0) {
+        fprintf(stderr, "argweaver_amd: tail traceback tile length %d out of range\n", cols);
+        return -1;
+    }
+    reinterpret_cast<AwBatch *>(b)->tb_tail = cols;
+    return 0;
+}
+
 int awamd_batch_run(void *b, int precision, int tile_cols, int traceback, double *stats)
 {
     return awamd_batch_run_opts(b, precision, tile_cols, traceback, 1, stats);
@@ -1020,6 +1032,7 @@ int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int
     opt.tile_cols = tile_cols;
     opt.tile_warm = ab.tb_warm;
     opt.tile_opts = ab.tb_opts;
+    opt.tail_tile_cols = ab.tb_tail;
     opt.traceback = traceback != 0;
     opt.fixup = fixup != 0;
     opt.fw_tile = fw_tile;
@@ -1028,11 +1041,11 @@ int awamd_batch_run_ex(void *b, int precision, int tile_cols, int traceback, int
     RunStats st;
     int rc = ab.batch.run(opt, &st);
     if (stats) {
-        const double v[12] = {st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback, st.ms_total,
+        const double v[14] = {st.ms_prep, st.ms_emit, st.ms_forward, st.ms_traceback, st.ms_total,
                               (double)st.W, (double)st.nvar, (double)st.tb_passes,
                               (double)st.tb_rerun_cols, (double)st.fw_tiles, (double)st.fw_reruns,
-                              st.ms_fw_main};
-        std::copy(v, v + 12, stats);
+                              st.ms_fw_main, st.ms_tb_tail, st.ms_tb_side};
+        std::copy(v, v + 14, stats);
     }
     return rc;
 }

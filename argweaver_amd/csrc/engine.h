@@ -109,6 +109,10 @@ struct RunStats {
     double ms_fw_main = 0;        // the main forward launch (without re-runs and checks)
     int fw_tiles = 0;             // forward tiles launched
     int fw_reruns = 0;            // forward tiles re-run from their boundary column
+    // with forward re-runs: from the end of the forward to the end of the
+    // re-run instances' traceback, and to the end of the others' (which
+    // started beside the re-runs; negative if it ended before the forward)
+    double ms_tb_tail = 0, ms_tb_side = 0;
 };
 
 // One threading instance described on the host.
@@ -135,6 +139,11 @@ struct BatchOptions {
     int tile_warm = 256;     // warm-up columns of a speculative tile start
     int tile_opts = 3;       // options per traceback tile (1 .. 4): the
                              // speculative chain and its alternatives
+    // traceback tile length of the instances whose FP32 forward tiles are
+    // re-run: their traceback runs after the forward, when the rest of the
+    // batch is done or on the other stream, so shorter tiles spread it over
+    // more of the device (0: tile_cols)
+    int tail_tile_cols = 4000;
     bool traceback = true;
     bool fixup = true;       // false: speculative option-0 paths only (diagnostics)
     // FP32 forward tiles: long instances are split into tiles (counting the

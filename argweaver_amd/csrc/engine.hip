@@ -43,6 +43,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -1289,7 +1291,8 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
         if (rejoined) break;
     }
     }
-    if (early && lane == 0) *early = rejoined ? 1 : 0;
+    // the column after the one that rejoined (never 0), else 0
+    if (early && lane == 0) *early = rejoined ? col + 1 : 0;
 }
 
 // One launch covers every instance of a batch, whatever its states-per-lane
@@ -1955,7 +1958,7 @@ std::recursive_mutex &device_mutex()
 struct Streams {
     bool init = false;
     hipStream_t main = nullptr, side = nullptr;
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];
     int create()
     {
         if (init) return 0;
@@ -2322,11 +2325,12 @@ inline double tb_cost_model(int R)
 }
 
 // Traceback tiles of every instance: about tile_cols columns (0: one per
-// instance; scaled by the column cost of the instance's R), the instances
-// with the most states per lane first, K options per tile.  Copies the tile
-// list to the device on the main stream; returns the LDS size of the tile
-// kernels (sht).
-static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K, size_t &sht)
+// instance; scaled by the column cost of the instance's R), tail_cols for
+// the instances with tail[inst] set, the instances with the most states per
+// lane first, K options per tile.  Copies the tile list to the device on the
+// main stream; returns the LDS size of the tile kernels (sht).
+static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K, size_t &sht,
+                      const std::vector<char> *tail = nullptr, int tail_cols = 0)
 {
     const int n = (int)m.h.size();
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
@@ -2344,8 +2348,9 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
         // tile_cols columns at R = 12's column cost; instances with cheaper
         // columns get proportionally longer tiles (fewer tiles, the same
         // time per tile)
+        const int tc = (tail && (*tail)[i] && tail_cols > 0) ? tail_cols : tile_cols;
         const int tile =
-            tile_cols > 0 ? std::max(64, (int)(tile_cols * tb_cost_model(12) / tb_cost_model(d.R))) : d.L;
+            tc > 0 ? std::max(64, (int)(tc * tb_cost_model(12) / tb_cost_model(d.R))) : d.L;
         const int nt = (d.L + tile - 1) / tile;
         for (int t = 0; t < nt; t++) {
             Chunk k;
@@ -2604,6 +2609,17 @@ inline double fw_cost_model(int R)
 // checked against the boundary column of the tile before, and tiles that
 // fail are re-run from that column until every check passes (a re-run
 // changes the boundary column of the next tile, which is checked again).
+// AWAMD_TRACE_TILES=1 in the environment: the forward re-runs are printed
+// to stderr (diagnostics)
+static bool trace_tiles()
+{
+    static const bool on = [] {
+        const char *e = getenv("AWAMD_TRACE_TILES");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
                          const std::vector<int> &order, size_t shf, int t1max, RunStats &rs,
                          const std::function<int(const std::vector<char> &)> &first_check)
@@ -2665,6 +2681,7 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         if (pass == 0 && m.fwb.ensure(p.used + 4096)) return 1;
     }
     HIP_OK(hipMemcpyAsync(m.d_fwt, tl.data(), sizeof(FwTile) * nt, hipMemcpyHostToDevice, s));
+    const auto t_fw = std::chrono::steady_clock::now();
     (void)hipEventRecord(c.ev[6], s);
     if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max)
              : launch_fw<double>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max))
@@ -2709,6 +2726,11 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
             sub.push_back(t);
         }
         rs.fw_reruns += (int)redo.size();
+        if (trace_tiles()) {
+            for (int k : redo)
+                fprintf(stderr, "fw re-run: inst %d (L %d, R %d) tile [%d, %d)\n", tl[k].inst,
+                        m.h[tl[k].inst].L, m.h[tl[k].inst].R, tl[k].cs, tl[k].c1);
+        }
         // the tile list keeps every tile's index for the checks; the re-run
         // tiles are launched from a list of their own
         const int nr = (int)sub.size();
@@ -2727,6 +2749,13 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
         chk.clear();
         for (int x = 0; x < nr; x++)
             if (!early[x] && next[redo[x]] >= 0) chk.push_back(next[redo[x]]);
+        if (trace_tiles()) {
+            for (int x = 0; x < nr; x++)
+                fprintf(stderr, "  inst %d tile [%d, %d): %s %d\n", sub[x].inst, sub[x].cs,
+                        sub[x].c1, early[x] ? "rejoined at" : "ran to", early[x] ? early[x] - 1 : sub[x].c1);
+            fprintf(stderr, "fw re-run round: %d tiles, %.1f ms since the forward was queued\n", nr,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_fw).count());
+        }
     }
     if (first) return first_check(std::vector<char>(m.h.size(), 0));
     return 0;
@@ -2922,13 +2951,17 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     auto early_tb = [&](const std::vector<char> &b) -> int {
         if (!opt.traceback || std::find(b.begin(), b.end(), 1) == b.end()) return 0;
         busy = b;
-        if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht)) return 1;
+        if (plan_tiles(m, fp32, opt.tile_cols, opt.tile_warm, K, sht, &b, opt.tail_tile_cols))
+            return 1;
         HIP_OK(hipStreamSynchronize(s));   // the tile list is on the device
         done.assign(n, 0);
         for (int i = 0; i < n; i++) done[i] = !b[i];
         if (launch_options(m, fp32, sht, &done, 0, c.side)) return 1;
         if (opt.fixup)
-            helper = std::thread([&] { rc_side = resolve_tiles(m, fp32, sht, rs_side, c.side, &done, 1); });
+            helper = std::thread([&] {
+                rc_side = resolve_tiles(m, fp32, sht, rs_side, c.side, &done, 1);
+                (void)hipEventRecord(c.ev[8], c.side);
+            });
         return 0;
     };
     // the helper is joined on every path out of run()
@@ -2950,6 +2983,7 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
         } else {
             if (launch_options(m, fp32, sht, &busy, 1, s)) return 1;
             if (opt.fixup && resolve_tiles(m, fp32, sht, rs, s, &busy, 0)) return 1;
+            (void)hipEventRecord(c.ev[9], s);
             if (helper.joinable()) helper.join();
             HIP_OK(hipStreamSynchronize(c.side));
             if (rc_side) return 1;
@@ -2976,6 +3010,12 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     (void)hipEventElapsedTime(&ms, c.ev[3], c.ev[5]);
     rs.ms_traceback = opt.traceback ? std::max(0.f, ms - fwd) : 0.f;
     (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[5]); rs.ms_total = ms;
+    if (opt.traceback && opt.fixup && !busy.empty()) {
+        // the two halves of the overlapped traceback, from the end of the
+        // forward: the re-run instances' (main stream) and the others' (side)
+        (void)hipEventElapsedTime(&ms, c.ev[4], c.ev[9]); rs.ms_tb_tail = ms;
+        (void)hipEventElapsedTime(&ms, c.ev[4], c.ev[8]); rs.ms_tb_side = ms;
+    }
     for (int i = 0; i < n; i++) {
         rs.W += m.W[i];
         rs.nvar += m.nvar[i];

@@ -375,8 +375,10 @@ int awamd_batch_run_opts(void *batch, int precision, int tile_cols, int tracebac
  * (<= 0: 25000) from a uniform vector, its first stored column is checked
  * against the column computed by the tile before at relative tolerance
  * fw_tol (<= 0: 1e-5), and a tile that fails is re-run from that column.
- * stats[12]: the nine of awamd_batch_run, then forward tiles, re-run tiles
- * and the ms of the main forward launch. */
+ * stats[14]: the nine of awamd_batch_run, then forward tiles, re-run tiles,
+ * the ms of the main forward launch, and, when forward tiles were re-run,
+ * the ms from the forward's end to the end of the re-run instances'
+ * traceback and of the other instances' traceback (0 otherwise). */
 int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback, int fixup,
                        int fw_tile, int fw_warm, double fw_tol, double *stats);
 /* Traceback tiles of later runs: warm-up columns of a tile's speculative
@@ -385,6 +387,11 @@ int awamd_batch_run_ex(void *batch, int precision, int tile_cols, int traceback,
  * a tile no option fits is re-run).  The sampled thread is the same for
  * every setting.  0 on success, -1 for a value out of range. */
 int awamd_batch_set_traceback(void *batch, int warm, int options);
+/* Traceback tile length of the instances whose FP32 forward tiles are
+ * re-run (their traceback runs after the forward: shorter tiles spread it
+ * over more of the device); 0: the run's tile length.  The thread does not
+ * depend on it.  -1 for a negative length. */
+int awamd_batch_set_tail_tiles(void *batch, int cols);
 /* Traceback-schedule autotuner: runs the forward once, times the traceback
  * for each candidate tile length (0: one tile per instance) and returns the
  * fastest (-1 on error); ms_out[ncand] receives the timings when given. */

@@ -165,21 +165,24 @@ def test_alone_equals_in_mix(aw, mix, prec, tile, fw):
         assert np.array_equal(one.alpha(0), b.alpha(i)), i
 
 
-@pytest.mark.parametrize("options,warm", [(1, 256), (2, 0), (3, 256), (4, 50)])
-def test_traceback_options_match_untiled(aw, mix, options, warm):
+@pytest.mark.parametrize("options,warm,tail", [(1, 256, 0), (2, 0, 60), (3, 256, 100), (4, 50, 4000)])
+def test_traceback_options_match_untiled(aw, mix, options, warm, tail):
     """Traceback tiles with options, resolved right to left (alternatives
     adopted, re-runs where none fits), give the thread of one tile per
     instance on the same table, bit for bit.  The FP32 forward runs as tiles
     with re-runs (a 2-column warm-up), so the options of the instances that
-    are final early run beside those re-runs."""
-    print("traceback options=%d warm=%d" % (options, warm), flush=True)
+    are final early run beside those re-runs; the re-run instances' own
+    traceback tiles have the tail length."""
+    print("traceback options=%d warm=%d tail=%d" % (options, warm, tail), flush=True)
     b = _batch(aw, mix)
     kw = dict(fw_tile=900, fw_warm=2, fw_tol=1e-5)
     b.run(1, 0, **kw)
     ref = [b.path(i) for i in range(len(mix))]
     b.set_traceback(warm, options)
+    b.set_tail_tiles(tail)
     st = b.run(1, 150, **kw)
     assert st["fw_reruns"] >= 1
+    assert st["ms_tb_tail"] > 0
     for i in range(len(mix)):
         assert np.array_equal(b.path(i), ref[i]), (i, np.mean(b.path(i) == ref[i]))
 

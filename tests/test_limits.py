@@ -35,3 +35,12 @@ def test_traceback_options_out_of_range(aw):
     for bad in ((256, 0), (256, 5), (-1, 3)):
         with pytest.raises(ValueError):
             b.set_traceback(*bad)
+
+
+def test_tail_tiles_out_of_range(aw):
+    # the re-run instances' traceback tile length: >= 0 (0: the run's)
+    b = aw.Batch()
+    b.set_tail_tiles(0)
+    b.set_tail_tiles(4000)
+    with pytest.raises(ValueError):
+        b.set_tail_tiles(-1)
