@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/argweaver_amd.h"
@@ -62,52 +63,142 @@ static int build_instance(const ArgTrees &tr, const double *times, int ntimes,
     return 0;
 }
 
-// sample_recombinations (recomb.cpp:151-235) for external threading
-static void sample_recombinations(const ArgTrees &tr, const Model &m, const int *path,
-                                  std::vector<int> &rpos, std::vector<int> &rnode,
-                                  std::vector<int> &rtime)
-{
-    const int T = m.T;
-    std::vector<int> snode, stime, par, c0, c1, age;
-    std::vector<int> cn, ct;
-    std::vector<double> probs;
-    int end = tr.start_coord;
-    for (size_t bi = 0; bi < tr.blocks.size(); bi++) {
-        const LBlock &B = tr.blocks[bi];
-        const LTree &lt = B.tree;
+// The per-block inputs of the recombination sampler: the tree as arrays,
+// its lineage counts and its states' (node, time) by state index.
+struct RecombTree {
+    std::vector<int> par, c0, c1, age, off;   // off[node]: first state of node
+    Lineages L;
+    TreeView view() const
+    {
+        int root = 0;
+        while (par[root] != -1) root++;
+        return TreeView{(int)par.size(), root, par.data(), c0.data(), c1.data(), age.data()};
+    }
+    void build(const LTree &lt, int T)
+    {
         const int N = lt.nnodes;
-        par.resize(N); c0.resize(N); c1.resize(N); age.resize(N);
+        par.resize(N); c0.resize(N); c1.resize(N); age.resize(N); off.resize(N + 1);
+        off[0] = 0;
         for (int i = 0; i < N; i++) {
             par[i] = lt.nodes[i].parent;
             c0[i] = lt.nodes[i].child[0];
             c1[i] = lt.nodes[i].child[1];
             age[i] = lt.nodes[i].age;
         }
-        TreeView t{N, lt.root, par.data(), c0.data(), c1.data(), age.data()};
-        Lineages L;
-        count_lineages(t, T, L);
-        TransVec tv;
-        calc_trans(t, m, L, tv);
-        tree_states(lt, T, snode, stime);
+        for (int i = 0; i < N; i++) {
+            const int top = par[i] == -1 ? T - 2 : age[par[i]];
+            off[i + 1] = off[i] + std::max(0, top - age[i] + 1);
+        }
+        lineages(T);
+    }
+    // count_lineages (local_tree.cpp:34-69) by a difference array over time:
+    // the same counts in O(nodes + times)
+    void lineages(int T)
+    {
+        int d[MAXT + 1] = {0};
+        for (int j = 0; j < T; j++) L.nb[j] = L.nr[j] = L.nc[j] = 0;
+        for (size_t i = 0; i < par.size(); i++) {
+            const int p = par[i], page = p == -1 ? T - 2 : age[p];
+            if (age[i] < page) {
+                d[age[i]]++;
+                d[page]--;
+            }
+            L.nr[page]++;
+            L.nc[page]++;
+            if (p == -1) L.nb[page]++;
+        }
+        for (int j = 0, run = 0; j < T; j++) {
+            run += d[j];
+            L.nb[j] += run;
+            L.nr[j] += run;
+            L.nc[j] += run;
+        }
+        L.nb[T - 1] = 1;
+    }
+    void state(int s, int &node, int &time) const
+    {
+        node = (int)(std::upper_bound(off.begin(), off.end(), s) - off.begin()) - 1;
+        time = age[node] + (s - off[node]);
+    }
+};
 
-        int start = end;
-        end += B.blocklen;
+// sample_recombinations (recomb.cpp:151-235) for external threading.  The
+// draws are sequential (one glibc rand() stream), but what they need per
+// column is fixed by the path: the rate of the waiting-time draw at every
+// column whose state repeats the previous one.  Those rates (a TransMatrix
+// per block) are computed first, over the blocks in parallel; the draws
+// then run in the reference's order, building a block's tree arrays only
+// where a recombination is placed.
+static void sample_recombinations(const ArgTrees &tr, const Model &m, const int *path,
+                                  std::vector<int> &rpos, std::vector<int> &rnode,
+                                  std::vector<int> &rtime)
+{
+    const int T = m.T, nb = (int)tr.blocks.size(), sc = tr.start_coord;
+    std::vector<int> bstart(nb + 1, sc);
+    for (int b = 0; b < nb; b++) bstart[b + 1] = bstart[b] + tr.blocks[b].blocklen;
+    std::vector<double> rate((size_t)tr.length(), 0.0);
+    auto rates = [&](int b0, int b1) {
+        RecombTree rt;
+        TransVec tv;
+        for (int b = b0; b < b1; b++) {
+            const int start = bstart[b], end = bstart[b + 1];
+            if (end - start < 2) continue;
+            rt.build(tr.blocks[b].tree, T);
+            calc_trans(rt.view(), m, rt.L, tv);
+            int last = -1;
+            double r = 0.0;
+            for (int i = start + 1; i < end; i++) {
+                const int pl = path[i - 1 - sc];
+                if (path[i - sc] != pl) continue;
+                if (pl != last) {
+                    int node, a;
+                    rt.state(pl, node, a);
+                    const double self = tv.get_time(a, a, rt.age[node], true);
+                    r = 1.0 - (tv.norecombs[a] / self);
+                    last = pl;
+                }
+                rate[i - sc] = r;
+            }
+        }
+    };
+    const int nthr = nb < 256 ? 1 : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nthr == 1) {
+        rates(0, nb);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthr; t++)
+            pool.emplace_back(rates, (int)((long long)nb * t / nthr), (int)((long long)nb * (t + 1) / nthr));
+        for (auto &th : pool) th.join();
+    }
+
+    RecombTree rt;
+    std::vector<int> cn, ct;
+    std::vector<double> probs;
+    for (int bi = 0; bi < nb; bi++) {
+        const int start = bstart[bi], end = bstart[bi + 1];
+        bool built = false;
         int next_recomb = -1;
         // the first column of every block is a switch (or the very first column)
         for (int i = start + 1; i < end; i++) {
-            const int pi = path[i - tr.start_coord], pl = path[i - 1 - tr.start_coord];
+            const int pi = path[i - sc], pl = path[i - 1 - sc];
             if (pi == pl) {
                 if (i > next_recomb) {
-                    int a = stime[pl];
-                    double self = tv.get_time(a, a, age[snode[pl]], true);
-                    double rate = 1.0 - (tv.norecombs[a] / self);
-                    double ev = -log(frand()) / rate;
+                    double ev = -log(frand()) / rate[i - sc];
                     next_recomb = int(fmin(double(end), i + ev));
                 }
                 if (i < next_recomb) continue;
             }
             next_recomb = -1;
-            const int sn = snode[pi], st = stime[pi], ln = snode[pl], ltm = stime[pl];
+            if (!built) {
+                rt.build(tr.blocks[bi].tree, T);
+                built = true;
+            }
+            const std::vector<int> &par = rt.par, &age = rt.age;
+            const Lineages &L = rt.L;
+            const int root = rt.view().root;
+            int sn, st, ln, ltm;
+            rt.state(pi, sn, st);
+            rt.state(pl, ln, ltm);
             // get_possible_recomb (recomb.cpp:124-148)
             cn.clear();
             ct.clear();
@@ -119,7 +210,7 @@ static void sample_recombinations(const ArgTrees &tr, const Model &m, const int 
             probs.clear();
             for (size_t x = 0; x < cn.size(); x++) {
                 int k = ct[x], j = st;
-                int root_time = age[lt.root] > ltm ? age[lt.root] : ltm;
+                int root_time = age[root] > ltm ? age[root] : ltm;
                 int rpa = (cn[x] == -1 || par[cn[x]] == -1 || cn[x] == ln) ? ltm : age[par[cn[x]]];
                 int nbk = L.nb[k] + int(k < ltm);
                 int nrk = L.nr[k] + int(k <= ltm) + int(k == ltm) - int(k == root_time);

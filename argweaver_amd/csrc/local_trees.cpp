@@ -285,6 +285,18 @@ void tree_states(const LTree &t, int ntimes, std::vector<int> &node, std::vector
     }
 }
 
+// AWAMD_CHECK_TREES=1 in the environment: every thread insertion and
+// removal is followed by the structural checks of the whole ARG (the
+// reference's assert_trees), which abort on a failure
+bool check_edits()
+{
+    static const bool on = [] {
+        const char *e = getenv("AWAMD_CHECK_TREES");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 // add_arg_thread (thread.cpp:272-418)
 void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
                     const std::vector<int> &recomb_pos, const std::vector<int> &recomb_node,
@@ -300,7 +312,16 @@ void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
     tr.seqids.push_back(seqid);
     tr.nnodes = nnodes2;
 
-    std::vector<int> snode, stime;
+    // (node, time) of a state of the block's tree before the branch is
+    // added: off[node] is the node's first state (node-major order)
+    std::vector<int> off, age0;
+    auto state_node = [&](int s) {
+        return (int)(std::upper_bound(off.begin(), off.end(), s) - off.begin()) - 1;
+    };
+    auto state_time = [&](int s) {
+        const int n = state_node(s);
+        return age0[n] + (s - off[n]);
+    };
     std::vector<LBlock> out;
     out.reserve(tr.blocks.size() + recomb_pos.size());
     const LTree *last_tree = nullptr;
@@ -311,21 +332,33 @@ void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
         LBlock *blk = &out.back();
         int start = end;
         end += blk->blocklen;
-        tree_states(blk->tree, ntimes, snode, stime);
+        {
+            const LTree &t = blk->tree;
+            off.resize(t.nnodes + 1);
+            age0.resize(t.nnodes);
+            off[0] = 0;
+            for (int i = 0; i < t.nnodes; i++) {
+                const int p = t.nodes[i].parent;
+                const int top = (p == -1) ? ntimes - 2 : t.nodes[p].age;
+                age0[i] = t.nodes[i].age;
+                off[i + 1] = off[i] + std::max(0, top - age0[i] + 1);
+            }
+        }
 
         blk->tree.ensure_capacity(nnodes2);
         if (!blk->mapping.empty()) blk->mapping.resize(std::max<size_t>(blk->mapping.size(), nnodes2));
-        int s0 = path[start - sc];
-        add_tree_branch(blk->tree, snode[s0], stime[s0]);
+        const int s0 = path[start - sc];
+        const int n0 = state_node(s0), t0 = state_time(s0);
+        add_tree_branch(blk->tree, n0, t0);
         if (!blk->mapping.empty())
-            add_spr_branch(blk->tree, *last_tree, snode[s0], stime[s0], last_node, blk->spr,
+            add_spr_branch(blk->tree, *last_tree, n0, t0, last_node, blk->spr,
                            blk->mapping.data(), newleaf, displaced, newcoal);
 
         for (; irecomb < recomb_pos.size() && recomb_pos[irecomb] < end; irecomb++) {
             int pos = recomb_pos[irecomb];
             const LTree &tree = blk->tree;
-            int sn = snode[path[pos - sc]], stt = stime[path[pos - sc]];
-            int ln = snode[path[pos - 1 - sc]], lt = stime[path[pos - 1 - sc]];
+            int sn = state_node(path[pos - sc]), stt = state_time(path[pos - sc]);
+            int ln = state_node(path[pos - 1 - sc]), lt = state_time(path[pos - 1 - sc]);
 
             Spr spr2;
             spr2.rn = recomb_node[irecomb];
@@ -365,11 +398,11 @@ void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
         // out may reallocate on the next push_back; reserve() above prevents it
         last_tree = &out.back().tree;
         int le = path[end - 1 - sc];
-        last_node = snode[le];
+        last_node = state_node(le);
         if (last_node == newleaf) last_node = displaced;
     }
     tr.blocks = std::move(out);
-    if (!check_trees(tr)) {
+    if (check_edits() && !check_trees(tr)) {
         fprintf(stderr, "argweaver_amd: add_arg_thread produced an invalid ARG\n");
         abort();
     }
@@ -481,7 +514,7 @@ void remove_arg_thread(ArgTrees &tr, int remove_seqid)
     tr.seqids.resize(nleaves - 1);
     tr.nnodes -= 2;
     remove_null_sprs(tr);
-    if (!check_trees(tr)) {
+    if (check_edits() && !check_trees(tr)) {
         fprintf(stderr, "argweaver_amd: remove_arg_thread produced an invalid ARG\n");
         abort();
     }

@@ -86,8 +86,10 @@ void remove_arg_thread(ArgTrees &tr, int seqid);
 // (remove_null_sprs, local_tree.cpp:409-461).
 void remove_null_sprs(ArgTrees &tr);
 
-// Structural checks run after every edit (assert_trees, local_tree.cpp:1763-1809).
+// Structural checks (assert_trees, local_tree.cpp:1763-1809); thread
+// insertion and removal run them when check_edits() (AWAMD_CHECK_TREES=1).
 bool check_trees(const ArgTrees &tr);
+bool check_edits();
 
 // Flat, device-friendly copy of the blocks: per block nnodes ints of
 // parent/child0/child1/age, 4 spr ints, nnodes mapping ints, start, length.

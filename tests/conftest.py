@@ -3,6 +3,10 @@ import sys
 
 import pytest
 
+# every thread insertion and removal in the tests is followed by the
+# structural checks of the whole ARG (abort on a failure)
+os.environ.setdefault("AWAMD_CHECK_TREES", "1")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))

@@ -162,13 +162,17 @@ def test_oracle_forward_matches_compiled_reference(aw, k, L, seed):
 
 
 @needs_ref
-@pytest.mark.parametrize("seed", [1, 2, 3, 4])
-def test_sample_thread_chain_matches_compiled_reference(aw, seed):
+@pytest.mark.parametrize("seed,k,L", [(1, 8, 3000), (2, 8, 3000), (3, 8, 3000), (4, 8, 3000),
+                                      (5, 24, 60000)])
+def test_sample_thread_chain_matches_compiled_reference(aw, seed, k, L):
     """oracle traceback + recombination sampling + this library's add_arg_thread
-    reproduce the reference's arghmm_sample_thread node for node."""
-    k, L = 8, 3000
+    reproduce the reference's arghmm_sample_thread node for node.  The last
+    case has enough local trees for the sampler's per-block rates to be
+    computed on several threads."""
     times, pops, rho, mu = model()
     tr, seqs = aw.simulate(k, L, times, pops, rho, mu, seed)
+    if L > 3000:
+        assert aw.get_local_trees_ntrees(tr) >= 256
     P, Ag, S, bl = aw.trees_arrays(tr)
     rt = O.ref_new_trees(P, Ag, S, bl)
     O.ref_remove_thread(rt, k - 1)
