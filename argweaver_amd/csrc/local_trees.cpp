@@ -297,110 +297,123 @@ bool check_edits()
     return on;
 }
 
-// add_arg_thread (thread.cpp:272-418)
+// add_arg_thread (thread.cpp:272-418).  Every input block becomes its own
+// run of output blocks (the block with the new branch, then one block per
+// recombination inside it), so the blocks are rebuilt in parallel; the SPR
+// and mapping of each run's first block, which read the last tree of the
+// run before, are rewritten in a second parallel pass once every tree is
+// final.
 void add_arg_thread(ArgTrees &tr, int ntimes, const int *path, int seqid,
                     const std::vector<int> &recomb_pos, const std::vector<int> &recomb_node,
                     const std::vector<int> &recomb_time)
 {
-    size_t irecomb = 0;
     const int nleaves = tr.num_leaves();
     const int nnodes = tr.nnodes;
     const int nnodes2 = nnodes + 2;
     const int newleaf = nleaves, displaced = nnodes, newcoal = nnodes + 1;
     const int sc = tr.start_coord;
+    const int nb = (int)tr.blocks.size();
 
     tr.seqids.push_back(seqid);
     tr.nnodes = nnodes2;
 
-    // (node, time) of a state of the block's tree before the branch is
-    // added: off[node] is the node's first state (node-major order)
-    std::vector<int> off, age0;
-    auto state_node = [&](int s) {
-        return (int)(std::upper_bound(off.begin(), off.end(), s) - off.begin()) - 1;
-    };
-    auto state_time = [&](int s) {
-        const int n = state_node(s);
-        return age0[n] + (s - off[n]);
-    };
-    std::vector<LBlock> out;
-    out.reserve(tr.blocks.size() + recomb_pos.size());
-    const LTree *last_tree = nullptr;
-    int last_node = 0;   // node of the state at the end of the previous block
-    int end = sc;
-    for (size_t bi = 0; bi < tr.blocks.size(); bi++) {
-        out.push_back(std::move(tr.blocks[bi]));
-        LBlock *blk = &out.back();
-        int start = end;
-        end += blk->blocklen;
-        {
-            const LTree &t = blk->tree;
-            off.resize(t.nnodes + 1);
-            age0.resize(t.nnodes);
-            off[0] = 0;
-            for (int i = 0; i < t.nnodes; i++) {
-                const int p = t.nodes[i].parent;
-                const int top = (p == -1) ? ntimes - 2 : t.nodes[p].age;
-                age0[i] = t.nodes[i].age;
-                off[i + 1] = off[i] + std::max(0, top - age0[i] + 1);
-            }
-        }
-
-        blk->tree.ensure_capacity(nnodes2);
-        if (!blk->mapping.empty()) blk->mapping.resize(std::max<size_t>(blk->mapping.size(), nnodes2));
-        const int s0 = path[start - sc];
-        const int n0 = state_node(s0), t0 = state_time(s0);
-        add_tree_branch(blk->tree, n0, t0);
-        if (!blk->mapping.empty())
-            add_spr_branch(blk->tree, *last_tree, n0, t0, last_node, blk->spr,
-                           blk->mapping.data(), newleaf, displaced, newcoal);
-
-        for (; irecomb < recomb_pos.size() && recomb_pos[irecomb] < end; irecomb++) {
-            int pos = recomb_pos[irecomb];
-            const LTree &tree = blk->tree;
-            int sn = state_node(path[pos - sc]), stt = state_time(path[pos - sc]);
-            int ln = state_node(path[pos - 1 - sc]), lt = state_time(path[pos - 1 - sc]);
-
-            Spr spr2;
-            spr2.rn = recomb_node[irecomb];
-            spr2.rt = recomb_time[irecomb];
-            if (spr2.rn == newleaf) spr2.rn = displaced;
-            if (spr2.rn == -1) {
-                spr2.rn = newleaf;
-                spr2.cn = sn;
-                if (spr2.cn == newleaf) spr2.cn = displaced;
-                if (sn == ln && stt > lt) spr2.cn = newcoal;
-            } else {
-                if (stt > lt) spr2.cn = tree.nodes[newleaf].parent;
-                else spr2.cn = newleaf;
-            }
-            spr2.ct = stt;
-
-            LBlock nb;
-            nb.mapping.resize(tree.nodes.size());
-            for (int j = 0; j < nnodes2; j++) nb.mapping[j] = j;
-            nb.mapping[tree.nodes[spr2.rn].parent] = -1;
-            nb.tree = tree;
-            nb.tree.nnodes = nnodes2;
-            apply_spr(nb.tree, spr2);
-            nb.spr = spr2;
-
-            int block_end;
-            if (irecomb < recomb_pos.size() - 1) block_end = std::min(recomb_pos[irecomb + 1], end);
-            else block_end = end;
-            blk->blocklen = pos - start;
-            nb.blocklen = block_end - pos;
-            out.push_back(std::move(nb));
-            blk = &out.back();
-            start = pos;
-        }
-
-        last_tree = nullptr;
-        // out may reallocate on the next push_back; reserve() above prevents it
-        last_tree = &out.back().tree;
-        int le = path[end - 1 - sc];
-        last_node = state_node(le);
-        if (last_node == newleaf) last_node = displaced;
+    // per input block: its columns, its recombinations [rfirst, rfirst+1)
+    // and its first output block
+    std::vector<int> bstart(nb + 1, sc), rfirst(nb + 1, 0), ofirst(nb + 1, 0);
+    for (int b = 0; b < nb; b++) bstart[b + 1] = bstart[b] + tr.blocks[b].blocklen;
+    for (int b = 0, r = 0; b <= nb; b++) {
+        while (b < nb && r < (int)recomb_pos.size() && recomb_pos[r] < bstart[b]) r++;
+        rfirst[b] = (b < nb) ? r : (int)recomb_pos.size();
+        ofirst[b] = b + rfirst[b];
     }
+    // first state (node, time) of each block and node of its last state
+    // (newleaf renamed), both in the tree before the branch is added
+    std::vector<int> n0(nb), t0(nb), lastn(nb);
+    std::vector<LBlock> out(nb + recomb_pos.size());
+
+    parallel_ranges(nb, [&](int lo, int hi) {
+        // (node, time) of a state: off[node] is the node's first state
+        std::vector<int> off, age0;
+        auto state_node = [&](int st) {
+            return (int)(std::upper_bound(off.begin(), off.end(), st) - off.begin()) - 1;
+        };
+        auto state_time = [&](int st) {
+            const int n = state_node(st);
+            return age0[n] + (st - off[n]);
+        };
+        for (int bi = lo; bi < hi; bi++) {
+            const int start0 = bstart[bi], end = bstart[bi + 1];
+            LBlock *blk = &out[ofirst[bi]];
+            *blk = std::move(tr.blocks[bi]);
+            {
+                const LTree &t = blk->tree;
+                off.resize(t.nnodes + 1);
+                age0.resize(t.nnodes);
+                off[0] = 0;
+                for (int i = 0; i < t.nnodes; i++) {
+                    const int p = t.nodes[i].parent;
+                    const int top = (p == -1) ? ntimes - 2 : t.nodes[p].age;
+                    age0[i] = t.nodes[i].age;
+                    off[i + 1] = off[i] + std::max(0, top - age0[i] + 1);
+                }
+            }
+            blk->tree.ensure_capacity(nnodes2);
+            if (!blk->mapping.empty()) blk->mapping.resize(std::max<size_t>(blk->mapping.size(), nnodes2));
+            const int s0 = path[start0 - sc];
+            n0[bi] = state_node(s0);
+            t0[bi] = state_time(s0);
+            const int ln = state_node(path[end - 1 - sc]);
+            lastn[bi] = (ln == newleaf) ? displaced : ln;
+            add_tree_branch(blk->tree, n0[bi], t0[bi]);
+
+            int start = start0;
+            for (int ir = rfirst[bi]; ir < rfirst[bi + 1]; ir++) {
+                const int pos = recomb_pos[ir];
+                const LTree &tree = blk->tree;
+                const int sn = state_node(path[pos - sc]), stt = state_time(path[pos - sc]);
+                const int pl = path[pos - 1 - sc];
+                const int lnode = state_node(pl), lt = state_time(pl);
+
+                Spr spr2;
+                spr2.rn = recomb_node[ir];
+                spr2.rt = recomb_time[ir];
+                if (spr2.rn == newleaf) spr2.rn = displaced;
+                if (spr2.rn == -1) {
+                    spr2.rn = newleaf;
+                    spr2.cn = sn;
+                    if (spr2.cn == newleaf) spr2.cn = displaced;
+                    if (sn == lnode && stt > lt) spr2.cn = newcoal;
+                } else {
+                    if (stt > lt) spr2.cn = tree.nodes[newleaf].parent;
+                    else spr2.cn = newleaf;
+                }
+                spr2.ct = stt;
+
+                LBlock &nbk = out[ofirst[bi] + 1 + (ir - rfirst[bi])];
+                nbk.mapping.resize(tree.nodes.size());
+                for (int j = 0; j < nnodes2; j++) nbk.mapping[j] = j;
+                nbk.mapping[tree.nodes[spr2.rn].parent] = -1;
+                nbk.tree = tree;
+                nbk.tree.nnodes = nnodes2;
+                apply_spr(nbk.tree, spr2);
+                nbk.spr = spr2;
+
+                const int block_end = (ir + 1 < (int)recomb_pos.size()) ? std::min(recomb_pos[ir + 1], end) : end;
+                blk->blocklen = pos - start;
+                nbk.blocklen = block_end - pos;
+                blk = &nbk;
+                start = pos;
+            }
+        }
+    });
+    parallel_ranges(nb, [&](int lo, int hi) {
+        for (int bi = std::max(lo, 1); bi < hi; bi++) {
+            LBlock &blk = out[ofirst[bi]];
+            if (blk.mapping.empty()) continue;
+            add_spr_branch(blk.tree, out[ofirst[bi] - 1].tree, n0[bi], t0[bi], lastn[bi - 1], blk.spr,
+                           blk.mapping.data(), newleaf, displaced, newcoal);
+        }
+    });
     tr.blocks = std::move(out);
     if (check_edits() && !check_trees(tr)) {
         fprintf(stderr, "argweaver_amd: add_arg_thread produced an invalid ARG\n");
@@ -441,7 +454,6 @@ void remove_arg_thread(ArgTrees &tr, int remove_seqid)
 {
     const int nnodes = tr.nnodes;
     const int nleaves = tr.num_leaves();
-    std::vector<int> displace(nnodes);
     const int last_leaf = nleaves - 1;
 
     int remove_leaf = -1;
@@ -463,52 +475,71 @@ void remove_arg_thread(ArgTrees &tr, int remove_seqid)
         return;
     }
 
-    for (size_t i = 0; i < tr.blocks.size(); i++) {
-        LTree &tree = tr.blocks[i].tree;
-        LNode *nodes = tree.nodes.data();
-        int remove_coal = nodes[remove_leaf].parent;
-        int coal_time = nodes[remove_coal].age;
-        int *c = nodes[remove_coal].child;
-        int coal_child = (c[0] == remove_leaf ? c[1] : c[0]);
-
-        remove_tree_branch(tree, remove_leaf, displace.data());
-
-        std::vector<int> &mapping = tr.blocks[i].mapping;
-        if (!mapping.empty())
-            for (int j = 0; j < nnodes - 2; j++)
-                mapping[j] = (mapping[j] != -1) ? displace[mapping[j]] : -1;
-
-        if (i + 1 >= tr.blocks.size()) continue;
-        std::vector<int> &m2 = tr.blocks[i + 1].mapping;
-        if (displace[last_leaf] != -1) m2[displace[last_leaf]] = m2[last_leaf];
-        if (displace[nnodes - 2] != -1) m2[displace[nnodes - 2]] = m2[nnodes - 2];
-        if (displace[nnodes - 1] != -1) m2[displace[nnodes - 1]] = m2[nnodes - 1];
-
-        Spr &spr = tr.blocks[i + 1].spr;
-        coal_child = displace[coal_child];
-        if (spr.rn == remove_leaf) {
-            spr = Spr{-1, -1, -1, -1};
-            continue;
+    // Each tree loses the branch on its own (first pass, in parallel); the
+    // renaming it implies (displace) then rewrites the keys of the next
+    // block's mapping and that block's SPR, and the values of the block's
+    // own mapping, in that order (second pass, each block reading the
+    // displacement of the block before)
+    const int nb = (int)tr.blocks.size();
+    struct Removed {
+        int coal, coal_time, coal_child;
+    };
+    std::vector<Removed> rem(nb);
+    std::vector<int> disp((size_t)nb * nnodes);
+    parallel_ranges(nb, [&](int lo, int hi) {
+        for (int i = lo; i < hi; i++) {
+            LTree &tree = tr.blocks[i].tree;
+            const LNode *nodes = tree.nodes.data();
+            Removed &r = rem[i];
+            r.coal = nodes[remove_leaf].parent;
+            r.coal_time = nodes[r.coal].age;
+            const int *c = nodes[r.coal].child;
+            r.coal_child = (c[0] == remove_leaf ? c[1] : c[0]);
+            remove_tree_branch(tree, remove_leaf, &disp[(size_t)i * nnodes]);
         }
-        if (spr.rn == remove_coal) spr.rn = coal_child;
-        else spr.rn = displace[spr.rn];
-        if (spr.rn == coal_child && nodes[coal_child].parent == -1) {
-            spr = Spr{-1, -1, -1, -1};
-            continue;
+    });
+    parallel_ranges(nb, [&](int lo, int hi) {
+        for (int i = lo; i < hi; i++) {
+            std::vector<int> &mapping = tr.blocks[i].mapping;
+            if (i > 0) {
+                // the previous block's renaming: keys of this mapping, this SPR
+                const int *displace = &disp[(size_t)(i - 1) * nnodes];
+                const Removed &r = rem[i - 1];
+                const LNode *nodes = tr.blocks[i - 1].tree.nodes.data();
+                std::vector<int> &m2 = mapping;
+                if (displace[last_leaf] != -1) m2[displace[last_leaf]] = m2[last_leaf];
+                if (displace[nnodes - 2] != -1) m2[displace[nnodes - 2]] = m2[nnodes - 2];
+                if (displace[nnodes - 1] != -1) m2[displace[nnodes - 1]] = m2[nnodes - 1];
+
+                Spr &spr = tr.blocks[i].spr;
+                const int coal_child = displace[r.coal_child];
+                if (spr.rn == remove_leaf) {
+                    spr = Spr{-1, -1, -1, -1};
+                } else {
+                    if (spr.rn == r.coal) spr.rn = coal_child;
+                    else spr.rn = displace[spr.rn];
+                    if (spr.rn == coal_child && nodes[coal_child].parent == -1) {
+                        spr = Spr{-1, -1, -1, -1};
+                    } else {
+                        if (spr.cn == remove_leaf) {
+                            spr.cn = coal_child;
+                            spr.ct = r.coal_time;
+                        } else if (spr.cn == r.coal) {
+                            spr.cn = coal_child;
+                        } else {
+                            spr.cn = displace[spr.cn];
+                        }
+                        if (spr.rn == spr.cn) spr = Spr{-1, -1, -1, -1};
+                    }
+                }
+            }
+            // this block's renaming: values of its mapping
+            const int *displace = &disp[(size_t)i * nnodes];
+            if (!mapping.empty())
+                for (int j = 0; j < nnodes - 2; j++)
+                    mapping[j] = (mapping[j] != -1) ? displace[mapping[j]] : -1;
         }
-        if (spr.cn == remove_leaf) {
-            spr.cn = coal_child;
-            spr.ct = coal_time;
-        } else if (spr.cn == remove_coal) {
-            spr.cn = coal_child;
-        } else {
-            spr.cn = displace[spr.cn];
-        }
-        if (spr.rn == spr.cn) {
-            spr = Spr{-1, -1, -1, -1};
-            continue;
-        }
-    }
+    });
 
     tr.seqids[remove_leaf] = tr.seqids[last_leaf];
     tr.seqids.resize(nleaves - 1);

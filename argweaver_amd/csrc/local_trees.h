@@ -8,7 +8,9 @@
 //   add/remove leaf src/argweaver/thread.cpp:9-554
 #pragma once
 
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "aw_math.h"
@@ -56,6 +58,22 @@ struct ArgTrees {
 };
 
 inline bool spr_is_null(const Spr &s) { return s.rn == -1; }
+
+// f(lo, hi) over consecutive ranges covering [0, n), on up to 16 host
+// threads (on the calling thread alone when n < 256)
+template <class F>
+void parallel_ranges(int n, F f)
+{
+    const int nthr = n < 256 ? 1 : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nthr == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthr; t++)
+        pool.emplace_back(f, (int)((long long)n * t / nthr), (int)((long long)n * (t + 1) / nthr));
+    for (auto &th : pool) th.join();
+}
 
 // Construct from the caller's arrays (LocalTrees ctor, local_tree.cpp:321-350).
 ArgTrees *trees_from_arrays(int **ptrees, int **ages, int **sprs, const int *blocklens,
