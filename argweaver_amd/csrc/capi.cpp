@@ -565,9 +565,17 @@ void arghmm_get_nstates(LocalTrees *trees, int ntimes, bool internal, int *nstat
     std::vector<int> sn, st;
     int i = 0;
     for (const LBlock &B : tr.blocks) {
-        if (internal) internal_states(B.tree, ntimes, 0, sn, st);
-        else tree_states(B.tree, ntimes, sn, st);
-        for (int j = 0; j < B.blocklen; j++) nstates[i + j] = (int)sn.size();
+        int S = 0;
+        if (internal) {
+            internal_states(B.tree, ntimes, 0, sn, st);
+            S = (int)sn.size();
+        } else {
+            for (int x = 0; x < B.tree.nnodes; x++) {
+                const int p = B.tree.nodes[x].parent;
+                S += (p == -1 ? ntimes - 2 : B.tree.nodes[p].age) - B.tree.nodes[x].age + 1;
+            }
+        }
+        for (int j = 0; j < B.blocklen; j++) nstates[i + j] = S;
         i += B.blocklen;
     }
 }
@@ -991,13 +999,18 @@ int awamd_thread_forward_noalloc(LocalTrees *trees, const double *times, int nti
 
 long long awamd_num_column_states(LocalTrees *trees, int ntimes, int *nstates_per_block)
 {
-    ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
-    FlatTrees ft;
-    flatten(tr, ft);
-    std::vector<int> Sv;
-    long long W = instance_states(ft, ntimes, &Sv);
-    if (nstates_per_block)
-        for (size_t i = 0; i < Sv.size(); i++) nstates_per_block[i] = Sv[i];
+    const ArgTrees &tr = *reinterpret_cast<ArgTrees *>(trees);
+    long long W = 0;
+    for (size_t b = 0; b < tr.blocks.size(); b++) {
+        const LTree &t = tr.blocks[b].tree;
+        int S = 0;
+        for (int i = 0; i < t.nnodes; i++) {
+            const int p = t.nodes[i].parent;
+            S += (p == -1 ? ntimes - 2 : t.nodes[p].age) - t.nodes[i].age + 1;
+        }
+        if (nstates_per_block) nstates_per_block[b] = S;
+        W += (long long)S * tr.blocks[b].blocklen;
+    }
     return W;
 }
 

@@ -645,23 +645,27 @@ void flatten(const ArgTrees &tr, FlatTrees &f)
     f.len.resize(B);
     int pos = 0;
     for (int b = 0; b < B; b++) {
-        const LBlock &blk = tr.blocks[b];
-        for (int i = 0; i < N; i++) {
-            const LNode &nd = blk.tree.nodes[i];
-            f.parent[(size_t)b * N + i] = nd.parent;
-            f.child0[(size_t)b * N + i] = nd.child[0];
-            f.child1[(size_t)b * N + i] = nd.child[1];
-            f.age[(size_t)b * N + i] = nd.age;
-            f.mapping[(size_t)b * N + i] = blk.mapping.empty() ? -1 : blk.mapping[i];
-        }
-        f.spr[4 * b + 0] = blk.spr.rn;
-        f.spr[4 * b + 1] = blk.spr.rt;
-        f.spr[4 * b + 2] = blk.spr.cn;
-        f.spr[4 * b + 3] = blk.spr.ct;
         f.start[b] = pos;
-        f.len[b] = blk.blocklen;
-        pos += blk.blocklen;
+        f.len[b] = tr.blocks[b].blocklen;
+        pos += f.len[b];
     }
+    parallel_ranges(B, [&](int lo, int hi) {
+        for (int b = lo; b < hi; b++) {
+            const LBlock &blk = tr.blocks[b];
+            for (int i = 0; i < N; i++) {
+                const LNode &nd = blk.tree.nodes[i];
+                f.parent[(size_t)b * N + i] = nd.parent;
+                f.child0[(size_t)b * N + i] = nd.child[0];
+                f.child1[(size_t)b * N + i] = nd.child[1];
+                f.age[(size_t)b * N + i] = nd.age;
+                f.mapping[(size_t)b * N + i] = blk.mapping.empty() ? -1 : blk.mapping[i];
+            }
+            f.spr[4 * b + 0] = blk.spr.rn;
+            f.spr[4 * b + 1] = blk.spr.rt;
+            f.spr[4 * b + 2] = blk.spr.cn;
+            f.spr[4 * b + 3] = blk.spr.ct;
+        }
+    });
 }
 
 }  // namespace aw

@@ -324,6 +324,7 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
         leaf = k - 1 - j
         t0 = time.perf_counter()
         A.remove_thread(tr, leaf)
+        t_rm = time.perf_counter() - t0
         uj = rng.uniform(size=L)
         if j == 0:
             # the first step's FP32 forward against FP64 (outside the timing)
@@ -345,20 +346,28 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
             log("mcmc fp32 step 0 vs fp64: |dlogL|/|logL| %.2e, alpha err abs %.2e rel %.2e"
                 % (err["rel_dlogL"], ea, er))
             del b
-            t0 = time.perf_counter()
+            t0 = time.perf_counter() - t_rm
+        t1 = time.perf_counter()
         r = A.thread_forward(tr, seqs, times, pops, rho, mu, leaf, uj,
                              precision=1, want_alpha=False)
+        t2 = time.perf_counter()
         pos, node, tm = A.sample_recombinations(tr, times, pops, rho, mu, r["path"])
+        t3 = time.perf_counter()
         A.add_thread(tr, len(times), r["path"], leaf, pos, node, tm)
-        dt = time.perf_counter() - t0
-        steps.append(dict(leaf=leaf, seconds=dt, lnl=r["lnl"], recombinations=int(len(pos)),
+        t4 = time.perf_counter()
+        dt = t4 - t0
+        steps.append(dict(leaf=leaf, seconds=dt, ms_remove=t_rm * 1e3,
+                          ms_thread_forward=(t2 - t1) * 1e3, ms_recombinations=(t3 - t2) * 1e3,
+                          ms_add=(t4 - t3) * 1e3, lnl=r["lnl"], recombinations=int(len(pos)),
                           local_trees=int(A.get_local_trees_ntrees(tr)),
                           log_likelihood=A.arg_likelihood(tr, seqs, times, mu),
                           log_prior=A.arg_prior(tr, times, pops, rho)))
-        log("mcmc fp32 re-thread leaf %d: %.2f s, thread logL %.6g, %d recombinations, %d local "
+        log("mcmc fp32 re-thread leaf %d: %.2f s (remove %.1f ms, thread_forward %.1f ms, "
+            "recombinations %.1f ms, add %.1f ms), thread logL %.6g, %d recombinations, %d local "
             "trees, log P(seqs | ARG) %.8g, log P(ARG) %.8g"
-            % (leaf, dt, r["lnl"], len(pos), steps[-1]["local_trees"],
-               steps[-1]["log_likelihood"], steps[-1]["log_prior"]))
+            % (leaf, dt, t_rm * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3, (t4 - t3) * 1e3, r["lnl"],
+               len(pos), steps[-1]["local_trees"], steps[-1]["log_likelihood"],
+               steps[-1]["log_prior"]))
     return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])),
                 fp32_error=err)
 
