@@ -1001,7 +1001,7 @@ struct FwTile {
     int mode;
     int bout, bin;
 };
-constexpr int BND_STRIDE = 64 * 16;   // Reals per boundary slot (the largest row)
+constexpr int BND_STRIDE = 64 * 32;   // Reals per boundary slot (the largest row)
 
 // wave-wide maximum
 __device__ __forceinline__ double wave_max(double x)
@@ -1301,8 +1301,11 @@ __device__ __forceinline__ void fw_instance(const DevInst d, Real *sm, const FwT
 // count: waves of different R run side by side from the start, with no
 // dependence on how streams map to hardware queues.
 // REDO marks the launches of re-run tiles (a separate symbol, so that
-// profiles report the main forward launch on its own)
-template <class Real, int NT1, bool REDO>
+// profiles report the main forward launch on its own).  BIG: the instances
+// with 32 states per lane (blocks of 1025-2048 states) run in a symbol of
+// their own, launched over the same tiles, so that their register demand
+// (spilled to AGPRs and scratch) does not weigh on the others.
+template <class Real, int NT1, bool REDO, bool BIG>
 __global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
                                                const FwTile *__restrict__ tiles,
                                                Real *__restrict__ bnd, double *__restrict__ bnorm,
@@ -1315,14 +1318,19 @@ __global__ __launch_bounds__(64) void k_fw_all(const DevInst *__restrict__ D,
     AW_GLOBAL Real *bg = (AW_GLOBAL Real *)bnd;
     AW_GLOBAL double *ng = (AW_GLOBAL double *)bnorm;
     AW_GLOBAL int *eg = early ? (AW_GLOBAL int *)(early + blockIdx.x) : nullptr;
-    switch (d.R) {
-    case 1: fw_instance<Real, 1, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    case 2: fw_instance<Real, 2, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    case 4: fw_instance<Real, 4, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    case 6: fw_instance<Real, 6, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    case 8: fw_instance<Real, 8, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    case 12: fw_instance<Real, 12, NT1>(d, sm, t, bg, ng, tol, eg); break;
-    default: fw_instance<Real, 16, NT1>(d, sm, t, bg, ng, tol, eg); break;
+    if ((d.R > 16) != BIG) return;   // the other symbol's tile
+    if constexpr (BIG) {
+        fw_instance<Real, 32, NT1>(d, sm, t, bg, ng, tol, eg);
+    } else {
+        switch (d.R) {
+        case 1: fw_instance<Real, 1, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        case 2: fw_instance<Real, 2, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        case 4: fw_instance<Real, 4, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        case 6: fw_instance<Real, 6, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        case 8: fw_instance<Real, 8, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        case 12: fw_instance<Real, 12, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        default: fw_instance<Real, 16, NT1>(d, sm, t, bg, ng, tol, eg); break;
+        }
     }
 }
 
@@ -1834,7 +1842,9 @@ __device__ __forceinline__ void tb_rerun(const DevInst d, const Chunk &ch, int k
 // (jj, o) of tile list[jj], option o sits at 8 * (K * (jj / 8) + o) + jj % 8,
 // so the options of a tile run on one XCD (blocks go round-robin over the
 // eight) and share its L2 for their rows.
-template <class Real>
+// BIG: the tiles of instances with 32 states per lane (a symbol of their own,
+// as for the forward)
+template <class Real, bool BIG>
 __global__ __launch_bounds__(64) void k_tb_spec(const DevInst *__restrict__ D,
                                                 const Chunk *__restrict__ chunks,
                                                 const int *__restrict__ list, int nlist, int K,
@@ -1849,19 +1859,24 @@ __global__ __launch_bounds__(64) void k_tb_spec(const DevInst *__restrict__ D,
     const Chunk ch = chunks[j];
     const DevInst &d = D[ch.inst];
     int *os = ostart + (size_t)j * K + o, *oe = oend + (size_t)j * K + o;
-    switch (d.R) {
-    case 1: tb_option<Real, 1>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    case 2: tb_option<Real, 2>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    case 4: tb_option<Real, 4>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    case 6: tb_option<Real, 6>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    case 8: tb_option<Real, 8>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    case 12: tb_option<Real, 12>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
-    default: tb_option<Real, 16>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+    if ((d.R > 16) != BIG) return;   // the other symbol's tile
+    if constexpr (BIG) {
+        tb_option<Real, 32>(d, ch, o, K, obuf, os, oe, tb_dyn);
+    } else {
+        switch (d.R) {
+        case 1: tb_option<Real, 1>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        case 2: tb_option<Real, 2>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        case 4: tb_option<Real, 4>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        case 6: tb_option<Real, 6>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        case 8: tb_option<Real, 8>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        case 12: tb_option<Real, 12>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        default: tb_option<Real, 16>(d, ch, o, K, obuf, os, oe, tb_dyn); break;
+        }
     }
 }
 
 // Re-runs: block x re-samples tile idx[x] from state k1[x] at its column c1.
-template <class Real>
+template <class Real, bool BIG>
 __global__ __launch_bounds__(64) void k_tb_rerun(const DevInst *__restrict__ D,
                                                  const Chunk *__restrict__ chunks,
                                                  const int *__restrict__ idx,
@@ -1877,14 +1892,19 @@ __global__ __launch_bounds__(64) void k_tb_rerun(const DevInst *__restrict__ D,
     const int *os = ostart + (size_t)j * K;
     int *rs = res + 3 * blockIdx.x;
     const int k = k1[blockIdx.x];
-    switch (d.R) {
-    case 1: tb_rerun<Real, 1>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    case 2: tb_rerun<Real, 2>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    case 4: tb_rerun<Real, 4>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    case 6: tb_rerun<Real, 6>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    case 8: tb_rerun<Real, 8>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    case 12: tb_rerun<Real, 12>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
-    default: tb_rerun<Real, 16>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+    if ((d.R > 16) != BIG) return;   // the other symbol's tile
+    if constexpr (BIG) {
+        tb_rerun<Real, 32>(d, ch, k, K, obuf, os, rs, tb_dyn);
+    } else {
+        switch (d.R) {
+        case 1: tb_rerun<Real, 1>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        case 2: tb_rerun<Real, 2>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        case 4: tb_rerun<Real, 4>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        case 6: tb_rerun<Real, 6>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        case 8: tb_rerun<Real, 8>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        case 12: tb_rerun<Real, 12>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        default: tb_rerun<Real, 16>(d, ch, k, K, obuf, os, rs, tb_dyn); break;
+        }
     }
 }
 
@@ -1908,8 +1928,8 @@ __global__ __launch_bounds__(256) void k_tb_adopt(const DevInst *__restrict__ D,
 namespace {
 
 // supported states-per-lane counts (one kernel instantiation each)
-constexpr int NRV = 7;
-constexpr int RVALS[NRV] = {1, 2, 4, 6, 8, 12, 16};
+constexpr int NRV = 8;
+constexpr int RVALS[NRV] = {1, 2, 4, 6, 8, 12, 16, 32};
 
 // Device-wide state, initialised once per process.
 struct Ctx {
@@ -2059,7 +2079,8 @@ int fw_lds_reals_r(bool fp32, int R)
     case 6: return fp32 ? fw_lds_reals<float, 6>() : fw_lds_reals<double, 6>();
     case 8: return fp32 ? fw_lds_reals<float, 8>() : fw_lds_reals<double, 8>();
     case 12: return fp32 ? fw_lds_reals<float, 12>() : fw_lds_reals<double, 12>();
-    default: return fp32 ? fw_lds_reals<float, 16>() : fw_lds_reals<double, 16>();
+    case 16: return fp32 ? fw_lds_reals<float, 16>() : fw_lds_reals<double, 16>();
+    default: return fp32 ? fw_lds_reals<float, 32>() : fw_lds_reals<double, 32>();
     }
 }
 
@@ -2078,12 +2099,12 @@ int set_lds_limit(const void *kern, size_t sh)
     return 0;
 }
 
-template <class Real>
+template <class Real, bool BIG = false>
 int launch_fw(const DevInst *dD, const FwTile *tiles, int cnt, void *bnd, double *bnorm,
               size_t sh, hipStream_t st, int T1max, double tol = 0.0, int *early = nullptr)
 {
-    auto kern = early ? ((T1max <= 19) ? k_fw_all<Real, 19, true> : k_fw_all<Real, MAXT - 1, true>)
-                      : ((T1max <= 19) ? k_fw_all<Real, 19, false> : k_fw_all<Real, MAXT - 1, false>);
+    auto kern = early ? ((T1max <= 19) ? k_fw_all<Real, 19, true, BIG> : k_fw_all<Real, MAXT - 1, true, BIG>)
+                      : ((T1max <= 19) ? k_fw_all<Real, 19, false, BIG> : k_fw_all<Real, MAXT - 1, false, BIG>);
     if (set_lds_limit((const void *)kern, sh)) return 1;
     hipLaunchKernelGGL(kern, dim3(cnt), dim3(64), sh, st, dD, tiles, (Real *)bnd, bnorm, tol, early);
     HIP_OK(hipGetLastError());
@@ -2126,6 +2147,10 @@ struct Batch::Impl {
     int *d_ridx = nullptr, *d_rk1 = nullptr, *d_rres = nullptr, *d_aidx = nullptr, *d_aopt = nullptr;
     int *d_tlist = nullptr;   // [2 nch] tile lists of the two option launches
     int tb_opts = 1;
+    // instances with 32 states per lane run in kernel symbols of their own;
+    // the LDS bytes of those symbols' forward and traceback launches
+    bool any_big = false;
+    size_t shf_big = 0, sht_big = 0;
     Chunk *d_chunks = nullptr;
     FwTile *d_fwt = nullptr, *d_fwre = nullptr;
     char *d_bnd = nullptr;
@@ -2271,8 +2296,8 @@ int Batch::upload()
                 }
                 const int R = lanes_per_state_r(Smax);
                 if (R < 0) {
-                    fprintf(stderr, "argweaver_amd: %d states per block exceed the 1024 of one wave\n",
-                            Smax);
+                    fprintf(stderr, "argweaver_amd: %d states per block exceed the %d of one wave\n",
+                            Smax, 64 * RVALS[NRV - 1]);
                     return 1;
                 }
                 d.hostprep = hi.prep ? 1 : 0;
@@ -2338,6 +2363,7 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
     const size_t rsz = fp32 ? sizeof(float) : sizeof(double);
     std::vector<Chunk> &chs = m.chunks;
     chs.clear();
+    m.sht_big = 0;
     // tiles of the instances with the most states per lane (the slowest
     // columns) are dispatched first
     std::vector<int> byr(n);
@@ -2369,7 +2395,8 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
         // (a,b) and (c,a,b) tables, then two row batches
         const int T1 = d.T - 1;
         const int bt = tb_batch_host(d.R, rsz);
-        sht = std::max(sht, rsz * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
+        size_t &sh = d.R > 16 ? m.sht_big : sht;
+        sh = std::max(sh, rsz * ((size_t)T1 * T1 + d.TC) + rsz * 2 * bt * 64 * (size_t)d.R);
     }
     const int nch = (int)chs.size();
     m.tb_opts = K;
@@ -2402,19 +2429,39 @@ static int launch_options(Batch::Impl &m, bool fp32, size_t sht, const std::vect
     const int nch = (int)m.chunks.size(), K = m.tb_opts;
     std::vector<int> &list = slot ? m.tlist1 : m.tlist0;
     list.clear();
-    for (int j = 0; j < nch; j++)
-        if (!take || (*take)[m.chunks[j].inst]) list.push_back(j);
+    // the tiles of the 32-states-per-lane instances after the others (a
+    // launch of their own symbol)
+    for (int big = 0; big < 2; big++)
+        for (int j = 0; j < nch; j++)
+            if ((!take || (*take)[m.chunks[j].inst]) && (m.h[m.chunks[j].inst].R > 16) == (big == 1))
+                list.push_back(j);
     const int nl = (int)list.size();
     if (!nl) return 0;
+    int nn = 0;
+    while (nn < nl && m.h[m.chunks[list[nn]].inst].R <= 16) nn++;
     int *dl = m.d_tlist + (size_t)slot * nch;
     HIP_OK(hipMemcpyAsync(dl, list.data(), sizeof(int) * nl, hipMemcpyHostToDevice, s));
-    const int grid = (nl + 7) / 8 * 8 * K;
-    return fp32 ? launch_lds(k_tb_spec<float>, grid, sht, s, (const DevInst *)m.dD,
-                             (const Chunk *)m.d_chunks, (const int *)dl, nl, K, m.d_obuf,
-                             m.d_ostart, m.d_oend)
-                : launch_lds(k_tb_spec<double>, grid, sht, s, (const DevInst *)m.dD,
-                             (const Chunk *)m.d_chunks, (const int *)dl, nl, K, m.d_obuf,
-                             m.d_ostart, m.d_oend);
+    if (nn > 0) {
+        const int grid = (nn + 7) / 8 * 8 * K;
+        if (fp32 ? launch_lds(k_tb_spec<float, false>, grid, sht, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)dl, nn, K, m.d_obuf,
+                              m.d_ostart, m.d_oend)
+                 : launch_lds(k_tb_spec<double, false>, grid, sht, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)dl, nn, K, m.d_obuf,
+                              m.d_ostart, m.d_oend))
+            return 1;
+    }
+    if (nl > nn) {
+        const int nb = nl - nn, grid = (nb + 7) / 8 * 8 * K;
+        if (fp32 ? launch_lds(k_tb_spec<float, true>, grid, m.sht_big, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)(dl + nn), nb, K, m.d_obuf,
+                              m.d_ostart, m.d_oend)
+                 : launch_lds(k_tb_spec<double, true>, grid, m.sht_big, s, (const DevInst *)m.dD,
+                              (const Chunk *)m.d_chunks, (const int *)(dl + nn), nb, K, m.d_obuf,
+                              m.d_ostart, m.d_oend))
+            return 1;
+    }
+    return 0;
 }
 
 // Resolution of the tiles, right to left within each instance.  A tile
@@ -2494,12 +2541,20 @@ static int resolve_tiles(Batch::Impl &m, bool fp32, size_t sht, RunStats &rs, hi
         const int np = (int)pj.size();
         HIP_OK(hipMemcpyAsync(ridx, pj.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(rk1, pk.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
-        if (fp32 ? launch_lds(k_tb_rerun<float>, np, sht, s, (const DevInst *)m.dD,
+        if (fp32 ? launch_lds(k_tb_rerun<float, false>, np, sht, s, (const DevInst *)m.dD,
                               (const Chunk *)m.d_chunks, (const int *)ridx,
                               (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres)
-                 : launch_lds(k_tb_rerun<double>, np, sht, s, (const DevInst *)m.dD,
+                 : launch_lds(k_tb_rerun<double, false>, np, sht, s, (const DevInst *)m.dD,
                               (const Chunk *)m.d_chunks, (const int *)ridx,
                               (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres))
+            return 1;
+        if (m.any_big &&
+            (fp32 ? launch_lds(k_tb_rerun<float, true>, np, m.sht_big, s, (const DevInst *)m.dD,
+                               (const Chunk *)m.d_chunks, (const int *)ridx,
+                               (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres)
+                  : launch_lds(k_tb_rerun<double, true>, np, m.sht_big, s, (const DevInst *)m.dD,
+                               (const Chunk *)m.d_chunks, (const int *)ridx,
+                               (const int *)rk1, K, m.d_obuf, (const int *)m.d_ostart, rres)))
             return 1;
         res.resize(3 * (size_t)np);
         HIP_OK(hipMemcpyAsync(res.data(), rres, sizeof(int) * res.size(), hipMemcpyDeviceToHost, s));
@@ -2688,6 +2743,10 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
     if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max)
              : launch_fw<double>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max))
         return 1;
+    if (m.any_big &&
+        (fp32 ? launch_fw<float, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, s, t1max)
+              : launch_fw<double, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, s, t1max)))
+        return 1;
     (void)hipEventRecord(c.ev[7], s);
     // tiles whose start is to be checked: those with a boundary slot in
     std::vector<int> chk;
@@ -2742,6 +2801,12 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
                                     opt.fw_tol, m.d_fflag)
                  : launch_fw<double>(m.dD, m.d_fwre, nr, m.d_bnd, m.d_bnorm, shf, s, t1max,
                                      opt.fw_tol, m.d_fflag))
+            return 1;
+        if (m.any_big &&
+            (fp32 ? launch_fw<float, true>(m.dD, m.d_fwre, nr, m.d_bnd, m.d_bnorm, m.shf_big, s,
+                                           t1max, opt.fw_tol, m.d_fflag)
+                  : launch_fw<double, true>(m.dD, m.d_fwre, nr, m.d_bnd, m.d_bnorm, m.shf_big, s,
+                                            t1max, opt.fw_tol, m.d_fflag)))
             return 1;
         // a tile that rejoined its stored columns kept its boundary column;
         // the tile after one that ran to its end is checked again
@@ -2932,10 +2997,14 @@ int Batch::run(const BatchOptions &opt, RunStats *st)
     HIP_OK(hipMemcpyAsync(m.d_list, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     size_t shf = 0;
     int t1max = 0;
+    m.any_big = false;
+    m.shf_big = 0;
     for (int i = 0; i < n; i++) {
         const DevInst &d = m.h[i];
         t1max = std::max(t1max, d.T - 1);
-        shf = std::max(shf, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
+        size_t &sh = d.R > 16 ? m.shf_big : shf;
+        sh = std::max(sh, rsz * (size_t)fw_lds_reals_r(fp32, d.R));
+        m.any_big |= d.R > 16;
     }
     RunStats rs;
     // Once the first check of the forward tiles has shown which instances

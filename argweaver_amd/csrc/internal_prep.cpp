@@ -584,13 +584,13 @@ int prepare_internal(const ArgTrees &tr, const Model &m, char **seqs, int minage
         Smax = std::max(Smax, hp.S[b]);
     }
     hp.R = -1;
-    for (int R : {1, 2, 4, 6, 8, 12, 16})
+    for (int R : {1, 2, 4, 6, 8, 12, 16, 32})
         if (64 * R >= Smax) {
             hp.R = R;
             break;
         }
     if (hp.R < 0) {
-        fprintf(stderr, "argweaver_amd: %d internal states in a block (limit 1024)\n", Smax);
+        fprintf(stderr, "argweaver_amd: %d internal states in a block (limit 2048)\n", Smax);
         return 1;
     }
     const int R = hp.R, RW = 64 * R;

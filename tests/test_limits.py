@@ -22,7 +22,7 @@ def test_batch_takes_instance_at_node_limit(aw):
     try:
         assert b.add(tr, seqs, *mdl, 128, None) == 0
     except RuntimeError:
-        # over 1024 states in a block is the other limit
+        # over 2048 states in a block is the other limit
         pytest.skip("instance exceeds the states-per-block limit")
     aw.delete_local_trees(tr)
 
