@@ -15,7 +15,7 @@
  * Errors: an invalid ARG aborts the process, as the reference's asserts
  * do.  Calls that run the HMM print to stderr and return NULL (or a
  * nonzero status) when no HIP device is visible, when an instance exceeds
- * the kernels' limits (256 nodes per tree, 32 time points, 1024 states per
+ * the kernels' limits (256 nodes per tree, 32 time points, 2048 states per
  * block) or when a device call fails, where the reference would abort.
  * Not thread-safe; the traceback draws from the process-global rand().
  * Device work of all calls is serialised inside the library.
