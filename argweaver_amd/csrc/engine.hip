@@ -2739,15 +2739,23 @@ static int forward_stage(Batch::Impl &m, bool fp32, const BatchOptions &opt,
     }
     HIP_OK(hipMemcpyAsync(m.d_fwt, tl.data(), sizeof(FwTile) * nt, hipMemcpyHostToDevice, s));
     const auto t_fw = std::chrono::steady_clock::now();
+    // the 32-states-per-lane tiles run on the side stream, beside the
+    // others: the host joins the main stream first (preparation, emissions
+    // and the tile list done), and the side stream before the checks
+    if (m.any_big) {
+        HIP_OK(hipStreamSynchronize(s));
+        if (fp32 ? launch_fw<float, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, c.side,
+                                          t1max)
+                 : launch_fw<double, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, c.side,
+                                           t1max))
+            return 1;
+    }
     (void)hipEventRecord(c.ev[6], s);
     if (fp32 ? launch_fw<float>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max)
              : launch_fw<double>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, shf, s, t1max))
         return 1;
-    if (m.any_big &&
-        (fp32 ? launch_fw<float, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, s, t1max)
-              : launch_fw<double, true>(m.dD, m.d_fwt, nt, m.d_bnd, m.d_bnorm, m.shf_big, s, t1max)))
-        return 1;
     (void)hipEventRecord(c.ev[7], s);
+    if (m.any_big) HIP_OK(hipStreamSynchronize(c.side));
     // tiles whose start is to be checked: those with a boundary slot in
     std::vector<int> chk;
     for (int k = 0; k < nt; k++)
