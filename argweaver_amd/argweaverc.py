@@ -208,6 +208,7 @@ awamd_batch_run_ex = _export(
     [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double_p])
 awamd_batch_set_traceback = _export("awamd_batch_set_traceback", c_int, [c_void_p, c_int, c_int])
 awamd_batch_set_tail_tiles = _export("awamd_batch_set_tail_tiles", c_int, [c_void_p, c_int])
+awamd_batch_clear = _export("awamd_batch_clear", c_int, [c_void_p])
 awamd_batch_tune = _export("awamd_batch_tune", c_int, [c_void_p, c_int, c_int_p, c_int, c_double_p])
 awamd_batch_tune_forward = _export(
     "awamd_batch_tune_forward", c_int, [c_void_p, c_int_p, c_int_p, c_int, c_double, c_double_p])
@@ -850,6 +851,12 @@ class Batch(object):
         self._keep.append((times, popsizes, cs, u))
         self.L.append(cs.seqlen)
         return i
+
+    def clear(self):
+        """Empty the batch for new instances; its device buffers are kept."""
+        awamd_batch_clear(self.ptr)
+        self._keep = []
+        self.L = []
 
     def upload(self):
         if awamd_batch_upload(self.ptr) != 0:

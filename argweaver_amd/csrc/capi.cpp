@@ -1059,7 +1059,13 @@ int awamd_batch_add(void *b, LocalTrees *trees, const double *times, int ntimes,
     if (build_instance(*reinterpret_cast<ArgTrees *>(trees), times, ntimes, popsizes, rho, mu,
                        seqs, new_chrom, hi))
         return -1;
-    return ab.batch.add(hi, uniforms);
+    return ab.batch.add(std::move(hi), uniforms);
+}
+
+int awamd_batch_clear(void *b)
+{
+    reinterpret_cast<AwBatch *>(b)->batch.clear();
+    return 0;
 }
 
 int awamd_batch_upload(void *b)

@@ -180,6 +180,8 @@ public:
     void set_concurrent(bool on);
     // the instance's row pointers and uniforms must stay valid until upload()
     int add(const HostInstance &hi, const double *u, const double *prior = nullptr);
+    // the same, taking the instance's arrays over instead of copying them
+    int add(HostInstance &&hi, const double *u, const double *prior = nullptr);
     // drop the instances, keep the device buffers for the next set
     void clear();
     int upload();

@@ -2229,6 +2229,11 @@ long long Batch::W(int i) const { return impl_->W[i]; }
 
 int Batch::add(const HostInstance &hi, const double *u, const double *prior)
 {
+    return add(HostInstance(hi), u, prior);
+}
+
+int Batch::add(HostInstance &&hin, const double *u, const double *prior)
+{
     Impl &m = *impl_;
     // upload() releases the host rows of the instances it copied, so a batch
     // that was uploaded takes no more instances until clear()
@@ -2237,11 +2242,12 @@ int Batch::add(const HostInstance &hi, const double *u, const double *prior)
         return -1;
     }
     std::string why;
-    if (instance_supported(hi, &why)) {
+    if (instance_supported(hin, &why)) {
         fprintf(stderr, "argweaver_amd: Batch::add refused the instance: %s\n", why.c_str());
         return -1;
     }
-    m.hosts.push_back(hi);
+    m.hosts.push_back(std::move(hin));
+    const HostInstance &hi = m.hosts.back();
     m.us.emplace_back();
     if (u) m.us.back().assign(u, u + hi.ft.seqlen);
     m.priors.emplace_back();

@@ -372,6 +372,80 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
                 fp32_error=err)
 
 
+def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
+    """Config 5 as batched FP32 chains: nchains independent chains, each a
+    k=100, 500k-column ARG simulated from its own seed under the same N(t).
+    Every step re-threads one leaf of every chain, the chains' instances in
+    one batch (reused across steps): remove the leaf (host), FP32 forward
+    with forward tiles and tiled traceback for all chains (GPU), then the
+    recombinations and the insertion per chain (host).  The leaves cycle
+    k-1, k-2, ..., so steps = k is one sweep re-threading every leaf of
+    every chain.  Reports the time per step, the aggregate column-states/s,
+    each chain's recombination count (local trees - 1) after every step and
+    its log P(seqs | ARG) and log P(ARG) before and after."""
+    times = np.array(A.get_time_points(20, 200e3), np.float64)
+    pops, rho, mu = mcmc_model(seed)
+    chains = [A.simulate(k, L, times, pops, rho, mu, seed + 100 + c) for c in range(nchains)]
+    rng = np.random.RandomState(seed + 31)
+    A.set_random_seed(seed + 32)
+    stats0 = [(A.arg_likelihood(tr, sq, times, mu), A.arg_prior(tr, times, pops, rho))
+              for tr, sq in chains]
+    b = A.Batch()
+    secs, Ws, trace, host_ms, gpu_ms = [], [], [], [], []
+    # the chains' host work runs on one thread per chain (the library's
+    # calls release the interpreter lock); the recombination draws of all
+    # chains come from the one process-global rand() stream, so they run in
+    # chain order
+    from concurrent.futures import ThreadPoolExecutor
+    pool = ThreadPoolExecutor(max_workers=nchains)
+    for j in range(steps):
+        leaf = k - 1 - (j % k)
+        t0 = time.perf_counter()
+        list(pool.map(lambda c: A.remove_thread(c[0], leaf), chains))
+        b.clear()
+        for tr, sq in chains:
+            b.add(tr, sq, times, pops, rho, mu, leaf, rng.uniform(size=L))
+        b.upload()
+        t1 = time.perf_counter()
+        st = b.run(1, 4096, fw_tile=50000, fw_warm=20000, fw_tol=1e-4)
+        t2 = time.perf_counter()
+        paths = [b.path(c) for c in range(nchains)]
+        recs = [A.sample_recombinations(tr, times, pops, rho, mu, paths[c])
+                for c, (tr, _) in enumerate(chains)]
+        list(pool.map(lambda c: A.add_thread(chains[c][0], len(times), paths[c], leaf, *recs[c]),
+                      range(nchains)))
+        t3 = time.perf_counter()
+        secs.append(t3 - t0)
+        Ws.append(st["W"])
+        host_ms.append(((t1 - t0) + (t3 - t2)) * 1e3)
+        gpu_ms.append((t2 - t1) * 1e3)
+        trace.append([int(A.get_local_trees_ntrees(tr)) - 1 for tr, _ in chains])
+        if j % 20 == 19 or j == steps - 1:
+            log("mcmc batched chains: step %d/%d, %.2f s per step (host %.0f ms, batch run %.0f "
+                "ms), recombinations %s" % (j + 1, steps, float(np.mean(secs[-20:])),
+                                            float(np.mean(host_ms[-20:])),
+                                            float(np.mean(gpu_ms[-20:])), trace[-1]))
+    pool.shutdown()
+    stats1 = [(A.arg_likelihood(tr, sq, times, mu), A.arg_prior(tr, times, pops, rho))
+              for tr, sq in chains]
+    for tr, _ in chains:
+        A.delete_local_trees(tr)
+    sps = float(np.mean(secs[1:])) if steps > 1 else secs[0]
+    out = dict(chains=nchains, k=k, columns=L, steps=steps,
+               leaves_rethreaded_per_chain=steps, seconds_per_step=sps,
+               host_ms_per_step=float(np.mean(host_ms[1:] or host_ms)),
+               batch_run_ms_per_step=float(np.mean(gpu_ms[1:] or gpu_ms)),
+               column_states_per_s=float(np.mean(Ws[1:] or Ws)) / sps,
+               recombinations_trace=trace,
+               log_likelihood_start=[x[0] for x in stats0], log_likelihood_end=[x[0] for x in stats1],
+               log_prior_start=[x[1] for x in stats0], log_prior_end=[x[1] for x in stats1])
+    log("mcmc batched chains: %d chains x %d steps, %.2f s per step, %.3g column-states/s, "
+        "log P(seqs | ARG) %s -> %s" % (nchains, steps, sps, out["column_states_per_s"],
+                                        ["%.8g" % x[0] for x in stats0],
+                                        ["%.8g" % x[0] for x in stats1]))
+    return out
+
+
 def mcmc_cpu_compare(A, seed, log, k=100, L=500000):
     """The reference chain beside the GPU one (CPU-baseline leg): the same
     config-5 ARG, one re-threading step from the same srand seed in the
@@ -509,6 +583,10 @@ def main():
                     help="Poisson request rates (req/s), comma-separated")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--no-mcmc", action="store_true", help="skip the config-5 re-threading sample")
+    ap.add_argument("--mcmc-chains", type=int, default=4,
+                    help="config-5 chains re-threaded together in one batch")
+    ap.add_argument("--mcmc-steps", type=int, default=100,
+                    help="steps of the batched config-5 chains (100: every leaf once; 0: skip)")
     ap.add_argument("--no-large", action="store_true", help="skip the config-2 single instance")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates HBM byte counters)")
@@ -635,6 +713,9 @@ def main():
     if not args.no_mcmc:
         extra["mcmc_rethread"] = mcmc_rethread(A, args.seed, log)
         extra["mcmc_rethread"]["fp32_chain"] = mcmc_rethread_fp32(A, args.seed, log)
+        if args.mcmc_steps > 0:
+            extra["mcmc_rethread"]["batched_chains"] = mcmc_batched_chains(
+                A, args.seed, log, nchains=args.mcmc_chains, steps=args.mcmc_steps)
 
     # dominant kernel: the FP32 forward's main launch (k_fw_all<float, 19,
     # false>, one per step; re-run tiles are a separate symbol); algorithmic

@@ -361,6 +361,10 @@ void awamd_batch_free(void *batch);
 int awamd_batch_add(void *batch, LocalTrees *trees, const double *times, int ntimes,
                     const double *popsizes, double rho, double mu, char **seqs, int new_chrom,
                     const double *uniforms);
+/* Empty the batch so that it takes new instances (its device buffers are
+ * kept and reused: a loop that re-fills one batch allocates only when it
+ * grows).  Returns 0. */
+int awamd_batch_clear(void *batch);
 int awamd_batch_upload(void *batch);
 int awamd_batch_run(void *batch, int precision, int tile_cols, int traceback, double *stats);
 /* as awamd_batch_run, with the traceback fix-up switch (0: speculative tiles

@@ -231,3 +231,25 @@ def test_fp32_forward_tiles_28_time_points(aw):
         assert abs(b.lnl(i) - it["lnl"]) <= 1e-6 * abs(it["lnl"]), (i, b.lnl(i), it["lnl"])
         assert _fp32_col_err(b, i, it) <= 2e-4, i
         assert np.mean(b.path(i) == it["path"]) >= 0.98, i
+
+
+def test_batch_clear_refill(aw, mix):
+    """A batch emptied with clear() and filled with other instances (its
+    device buffers reused) gives what a fresh batch of them gives, in both
+    precisions; the FP64 threads equal the oracle's."""
+    print("batch clear and refill", flush=True)
+    b = _batch(aw, mix[:2])
+    b.run(1, 300, fw_tile=900, fw_warm=200, fw_tol=1e-5)
+    b.clear()
+    for it in mix[2:]:
+        b.add(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+    b.upload()
+    fresh = _batch(aw, mix[2:])
+    for prec in (0, 1):
+        b.run(prec, 300, fw_tile=900, fw_warm=200, fw_tol=1e-5)
+        fresh.run(prec, 300, fw_tile=900, fw_warm=200, fw_tol=1e-5)
+        for i, it in enumerate(mix[2:]):
+            assert b.lnl(i) == fresh.lnl(i), (prec, i)
+            assert np.array_equal(b.path(i), fresh.path(i)), (prec, i)
+            if prec == 0:
+                assert np.array_equal(b.path(i), it["path"]), i
