@@ -2370,6 +2370,17 @@ static int plan_tiles(Batch::Impl &m, bool fp32, int tile_cols, int warm, int K,
     std::vector<Chunk> &chs = m.chunks;
     chs.clear();
     m.sht_big = 0;
+    // The tail instances' tiles run when the rest of the device is idle:
+    // tail_cols long, or longer when that would give more option waves
+    // than the device has SIMDs (each wave a SIMD to itself), since a
+    // second round of waves costs more than longer tiles do
+    if (tail && tail_cols > 0) {
+        double cols = 0;
+        for (int i = 0; i < n; i++)
+            if ((*tail)[i]) cols += (double)m.h[i].L * tb_cost_model(m.h[i].R) / tb_cost_model(12);
+        const double simds = 4.0 * std::max(1, ctx().prop.multiProcessorCount);
+        tail_cols = std::max(tail_cols, (int)std::ceil(cols * K / simds));
+    }
     // tiles of the instances with the most states per lane (the slowest
     // columns) are dispatched first
     std::vector<int> byr(n);
