@@ -1040,7 +1040,6 @@ int awamd_device_info(char *buf, int n) { return device_info(buf, n); }
 
 struct AwBatch {
     Batch batch;
-    std::vector<HostInstance> staged;
     int tb_warm = BatchOptions().tile_warm;
     int tb_opts = BatchOptions().tile_opts;
     int tb_tail = BatchOptions().tail_tile_cols;
