@@ -231,3 +231,34 @@ def test_resample_arg_region_changes_the_arg(aw):
         O.ref_delete(rt)
         aw.delete_local_trees(tr)
     assert changed >= 1
+
+
+def test_internal_big_blocks_match_reference(aw):
+    """Internal threading with blocks of more than 1024 internal states (128
+    haplotypes under a population size of 3e4; 32 states per lane, the
+    kernels' symbols for such instances, planes prepared on the host): the
+    forward table within 1e-12 of the compiled reference's and the same
+    sampled thread."""
+    k, L, seed = 128, 1200, 7
+    print("internal big blocks k=%d L=%d seed=%d" % (k, L, seed), flush=True)
+    tr, rt, seqs, (times, pops, rho, mu) = _pair(aw, k, L, seed, pops=np.full(20, 3e4))
+    _remove_path(aw, tr, rt, "leaf", 0, 400 + seed, len(times))
+    mine, ns = _forward_internal(aw._lib, tr, seqs, times, pops, rho, mu)
+    ref, rns = _forward_internal(O.ref(), rt, seqs, times, pops, rho, mu)
+    assert np.array_equal(ns, rns)
+    assert ns.max() > 1024, ns.max()
+    for c in range(L):
+        rel = np.abs(mine[c] - ref[c]) / np.maximum(np.abs(ref[c]), 1e-300)
+        assert rel.max() <= 1e-12, (c, int(ns[c]), rel.max())
+    s = O._Seqs(seqs)
+    p1 = np.zeros(L, np.int32)
+    p2 = np.zeros(L, np.int32)
+    args = (_d(times), c_int(len(times)), _d(pops), c_double(rho), c_double(mu), s.ptr,
+            c_int(s.arr.shape[0]), c_int(L))
+    O.srand(500 + seed)
+    aw._lib["arghmm_sample_arg_thread_internal"](c_void_p(tr), *args, _i(p1))
+    O.srand(500 + seed)
+    O.ref()["arghmm_sample_arg_thread_internal"](c_void_p(rt), *args, _i(p2))
+    assert np.array_equal(p1, p2)
+    O.ref_delete(rt)
+    aw.delete_local_trees(tr)
