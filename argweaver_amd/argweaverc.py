@@ -173,6 +173,8 @@ awamd_get_seqids = _export("awamd_get_seqids", c_int, [c_void_p, c_int_p])
 # ARG probability (argweaverc.py:218-241 of the reference binding)
 argweaver_likelihood = _export(
     "arghmm_likelihood", c_double, [c_void_p, c_double_p, c_int, c_double, c_char_p_p, c_int, c_int])
+awamd_arg_likelihood = _export(
+    "awamd_arg_likelihood", c_double, [c_void_p, c_double_p, c_int, c_double, c_char_p_p, c_int, c_int])
 argweaver_prior_prob = _export(
     "arghmm_prior_prob", c_double, [c_void_p, c_double_p, c_int, c_double_p, c_double])
 argweaver_tree_prior_prob = _export(
@@ -542,6 +544,14 @@ def arg_likelihood(trees, seqs, times, mu):
     times = np.ascontiguousarray(times, np.float64)
     cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
     return argweaver_likelihood(trees, _dptr(times), len(times), mu, cs.ptr, cs.nseqs, cs.seqlen)
+
+
+def arg_likelihood_device(trees, seqs, times, mu):
+    """log P(sequences | ARG) computed on the device (awamd_arg_likelihood):
+    the terms of arghmm_likelihood in another summation order."""
+    times = np.ascontiguousarray(times, np.float64)
+    cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
+    return awamd_arg_likelihood(trees, _dptr(times), len(times), mu, cs.ptr, cs.nseqs, cs.seqlen)
 
 
 def arg_prior(trees, times, popsizes, rho):

@@ -368,8 +368,22 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
             % (leaf, dt, t_rm * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3, (t4 - t3) * 1e3, r["lnl"],
                len(pos), steps[-1]["local_trees"], steps[-1]["log_likelihood"],
                steps[-1]["log_prior"]))
+    # the chain statistic log P(seqs | ARG) on the host (the reference's
+    # order of terms) and on the device (awamd_arg_likelihood)
+    t0 = time.perf_counter()
+    lh = A.arg_likelihood(tr, seqs, times, mu)
+    t1 = time.perf_counter()
+    A.arg_likelihood_device(tr, seqs, times, mu)   # first call: module load
+    t2 = time.perf_counter()
+    ld = A.arg_likelihood_device(tr, seqs, times, mu)
+    t3 = time.perf_counter()
+    lik = dict(host_s=t1 - t0, device_s=t3 - t2, device_first_call_s=t2 - t1,
+               rel_diff=abs(ld - lh) / abs(lh))
+    log("log P(seqs | ARG) of the config-5 ARG: host %.3f s, device %.3f s (first call %.3f s), "
+        "relative difference %.1e" % (lik["host_s"], lik["device_s"], lik["device_first_call_s"],
+                                       lik["rel_diff"]))
     return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])),
-                fp32_error=err)
+                fp32_error=err, arg_likelihood=lik)
 
 
 def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):

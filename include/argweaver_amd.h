@@ -142,6 +142,12 @@ void setLogLevel(int level);
  * prior of the first tree.  src/argweaver/total_prob.cpp:317-326 */
 double arghmm_likelihood(LocalTrees *trees, double *times, int ntimes, double mu, char **seqs,
                          int nseqs, int seqlen);
+/* log P(sequences | ARG) computed on the device (no counterpart in the
+ * reference; the same terms as arghmm_likelihood, calc_arg_likelihood,
+ * total_prob.cpp:19-45, summed in another order: within 1e-11 relative).
+ * NAN without a HIP device or for an ARG over the kernels' limits. */
+double awamd_arg_likelihood(LocalTrees *trees, const double *times, int ntimes, double mu,
+                            char **seqs, int nseqs, int seqlen);
 /* src/argweaver/total_prob.cpp:329-342: aborts, as the reference's does */
 double arghmm_likelihood_parsimony(LocalTrees *trees, double *times, int ntimes, double mu,
                                    char **seqs, int nseqs, int seqlen);
