@@ -137,3 +137,25 @@ def test_batch_add_after_upload_refused(aw):
     b.run(1, 0)
     path, lnl = _alone(aw, it, 1, 0)
     assert b.lnl(0) == lnl and np.array_equal(b.path(0), path)
+
+
+def test_served_big_block_request_equals_alone(aw):
+    """A request with blocks of more than 1024 states (32 states per lane,
+    the kernels' separate symbols) served beside small ones: every result
+    equals the request run alone."""
+    print("serve with a big-block request", flush=True)
+    items = []
+    for k, L, seed, pops in [(128, 1200, 91, 3e4), (8, 900, 92, 1e4), (20, 1500, 93, 1e4)]:
+        tr, seqs, _, model = instance(aw, k, L, seed, popsizes=np.full(20, pops))
+        u = np.random.RandomState(seed).uniform(size=L)
+        items.append(dict(k=k, trees=tr, seqs=seqs, model=model, u=u))
+    assert aw.num_column_states(items[0]["trees"], 20)[1].max() > 1024
+    srv = aw.Server(hbm_budget_gb=8.0, precision=1, tile_cols=300, max_batch=4)
+    ids = [srv.submit(it["trees"], it["seqs"], *it["model"], it["k"] - 1, it["u"])
+           for it in items]
+    res = [srv.wait(i, timeout=120) for i in ids]
+    srv.close()
+    for it, r in zip(items, res):
+        path, lnl = _alone(aw, it, 1, 300)
+        assert r["lnl"] == lnl
+        assert np.array_equal(r["path"], path)
