@@ -395,8 +395,9 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
     recombinations and the insertion per chain (host).  The leaves cycle
     k-1, k-2, ..., so steps = k is one sweep re-threading every leaf of
     every chain.  Reports the time per step, the aggregate column-states/s,
-    each chain's recombination count (local trees - 1) after every step and
-    its log P(seqs | ARG) and log P(ARG) before and after."""
+    each chain's recombination count (local trees - 1) and log P(seqs | ARG)
+    (computed on the device, outside the step's time) after every step, and
+    its log P(seqs | ARG) and log P(ARG) on the host before and after."""
     times = np.array(A.get_time_points(20, 200e3), np.float64)
     pops, rho, mu = mcmc_model(seed)
     chains = [A.simulate(k, L, times, pops, rho, mu, seed + 100 + c) for c in range(nchains)]
@@ -405,7 +406,7 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
     stats0 = [(A.arg_likelihood(tr, sq, times, mu), A.arg_prior(tr, times, pops, rho))
               for tr, sq in chains]
     b = A.Batch()
-    secs, Ws, trace, host_ms, gpu_ms = [], [], [], [], []
+    secs, Ws, trace, host_ms, gpu_ms, lik_trace = [], [], [], [], [], []
     # the chains' host work runs on one thread per chain (the library's
     # calls release the interpreter lock); the recombination draws of all
     # chains come from the one process-global rand() stream, so they run in
@@ -434,6 +435,9 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
         host_ms.append(((t1 - t0) + (t3 - t2)) * 1e3)
         gpu_ms.append((t2 - t1) * 1e3)
         trace.append([int(A.get_local_trees_ntrees(tr)) - 1 for tr, _ in chains])
+        # log P(seqs | ARG) of every chain after the step, on the device
+        # (outside the step's time)
+        lik_trace.append([A.arg_likelihood_device(tr, sq, times, mu) for tr, sq in chains])
         if j % 20 == 19 or j == steps - 1:
             log("mcmc batched chains: step %d/%d, %.2f s per step (host %.0f ms, batch run %.0f "
                 "ms), recombinations %s" % (j + 1, steps, float(np.mean(secs[-20:])),
@@ -450,7 +454,9 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
                host_ms_per_step=float(np.mean(host_ms[1:] or host_ms)),
                batch_run_ms_per_step=float(np.mean(gpu_ms[1:] or gpu_ms)),
                column_states_per_s=float(np.mean(Ws[1:] or Ws)) / sps,
-               recombinations_trace=trace,
+               recombinations_trace=trace, log_likelihood_trace=lik_trace,
+               log_likelihood_device_vs_host_end=[
+                   abs(lik_trace[-1][c] - stats1[c][0]) / abs(stats1[c][0]) for c in range(nchains)],
                log_likelihood_start=[x[0] for x in stats0], log_likelihood_end=[x[0] for x in stats1],
                log_prior_start=[x[1] for x in stats0], log_prior_end=[x[1] for x in stats1])
     log("mcmc batched chains: %d chains x %d steps, %.2f s per step, %.3g column-states/s, "
