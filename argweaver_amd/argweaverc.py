@@ -175,6 +175,8 @@ argweaver_likelihood = _export(
     "arghmm_likelihood", c_double, [c_void_p, c_double_p, c_int, c_double, c_char_p_p, c_int, c_int])
 awamd_arg_likelihood = _export(
     "awamd_arg_likelihood", c_double, [c_void_p, c_double_p, c_int, c_double, c_char_p_p, c_int, c_int])
+awamd_arg_prior = _export(
+    "awamd_arg_prior", c_double, [c_void_p, c_double_p, c_int, c_double_p, c_double])
 argweaver_prior_prob = _export(
     "arghmm_prior_prob", c_double, [c_void_p, c_double_p, c_int, c_double_p, c_double])
 argweaver_tree_prior_prob = _export(
@@ -544,6 +546,14 @@ def arg_likelihood(trees, seqs, times, mu):
     times = np.ascontiguousarray(times, np.float64)
     cs = seqs if isinstance(seqs, CSeqs) else CSeqs(seqs)
     return argweaver_likelihood(trees, _dptr(times), len(times), mu, cs.ptr, cs.nseqs, cs.seqlen)
+
+
+def arg_prior_device(trees, times, popsizes, rho):
+    """log P(ARG) computed on the device (awamd_arg_prior): the terms of
+    arghmm_prior_prob in another summation order."""
+    times = np.ascontiguousarray(times, np.float64)
+    popsizes = np.ascontiguousarray(popsizes, np.float64)
+    return awamd_arg_prior(trees, _dptr(times), len(times), _dptr(popsizes), rho)
 
 
 def arg_likelihood_device(trees, seqs, times, mu):

@@ -1,5 +1,5 @@
-// stats_gpu.hip -- log P(sequences | ARG) on the device: the chain statistic
-// an MCMC run reports every iteration (calc_arg_likelihood,
+// stats_gpu.hip -- the chain statistics an MCMC run reports every iteration,
+// on the device: log P(sequences | ARG) (calc_arg_likelihood,
 // src/argweaver/total_prob.cpp:19-45, with likelihood_tree,
 // src/argweaver/emit.cpp:399-449).
 //
@@ -279,6 +279,124 @@ int arg_loglik_device(const ArgTrees &tr, const double *times, int ntimes, doubl
     return 0;
 }
 
+// log P(ARG) (calc_arg_prior, src/argweaver/total_prob.cpp:262-299): a
+// term per local tree, each on a thread of its own: the probability of the
+// tree's recombination-free stretch and, when a tree follows, of the SPR
+// into it (calc_spr_prob, total_prob.cpp:210-257).
+struct PriorArgs {
+    int B, N;
+    const Model *m;
+    const int *parent, *c0, *c1, *age;   // [B*N]
+    const int *spr;                      // [4B] the SPR into block b
+    const int *len;                      // [B]
+    const int *has_next;                 // [B] a block follows before the end
+    double *term;                        // [B]
+    double *part;                        // [ceil(B/256)]
+};
+
+__device__ double spr_logprob_dev(const TreeView &t, const Model &m, int rn, int rt, int ct,
+                                  double treelen)
+{
+    const int T = m.T;
+    const int root_age = t.age[t.root];
+    const double treelen_b = treelen + m.steps[root_age];
+    Lineages L;
+    count_lineages(t, T, L);
+    L.nr[root_age]--;
+    double lnl = 0.0;
+    const int k = rt;
+    lnl += log(L.nb[k] * m.steps[k] / (L.nr[k] * treelen_b));
+    const int broken_age = t.age[t.parent[rn]];
+    const int j = ct;
+    auto rate = [&](int i) {
+        const int nb = L.nb[i / 2] - int(i / 2 < broken_age);
+        return m.csteps[i] * nb / (2.0 * m.pop[i / 2]);
+    };
+    const int ncoals_j = L.nc[j] - int(j <= broken_age) - int(j == broken_age);
+    lnl -= log((double)ncoals_j);
+    if (j < T - 2) lnl += log(1.0 - exp(-rate(2 * j) - (j > k ? rate(2 * j - 1) : 0.0)));
+    for (int x = 2 * k; x < 2 * j - 1; x++) lnl -= rate(x);
+    return lnl;
+}
+
+__global__ __launch_bounds__(256) void k_prior_block(PriorArgs a)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= a.B) return;
+    const int N = a.N;
+    const int *par = a.parent + (size_t)b * N;
+    int root = 0;
+    while (par[root] != -1) root++;
+    const TreeView t{N, root, par, a.c0 + (size_t)b * N, a.c1 + (size_t)b * N,
+                     a.age + (size_t)b * N};
+    const Model &m = *a.m;
+    const double treelen = tree_length(t, m.times);
+    const double rr = fmax(m.rho * treelen, m.rho);
+    double v;
+    if (a.has_next[b]) {
+        const int *sp = a.spr + 4 * (size_t)(b + 1);
+        v = log(rr) - rr * a.len[b] + spr_logprob_dev(t, m, sp[0], sp[1], sp[3], treelen);
+    } else {
+        v = -rr * a.len[b];
+    }
+    a.term[b] = v;
+}
+
+__global__ __launch_bounds__(256) void k_sum_terms(const double *term, int n, double *part)
+{
+    __shared__ double s[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    s[threadIdx.x] = i < n ? term[i] : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+int arg_logprior_device(const ArgTrees &tr, const Model &model, double *out)
+{
+    FlatTrees ft;
+    flatten(tr, ft);
+    const int B = ft.nblocks, N = ft.nnodes;
+    std::vector<int> has_next(B);
+    long long end = tr.start_coord;
+    for (int b = 0; b < B; b++) {
+        end += ft.len[b];
+        has_next[b] = end < tr.end_coord ? 1 : 0;
+    }
+    DevBufs m;
+    Model *dm;
+    int *dp, *dc0, *dc1, *dag, *dspr, *dlen, *dnext;
+    double *dterm, *dpart;
+    const int nwg = (B + 255) / 256;
+    if (m.alloc(dm, 1) || m.alloc(dp, (size_t)B * N) || m.alloc(dc0, (size_t)B * N) ||
+        m.alloc(dc1, (size_t)B * N) || m.alloc(dag, (size_t)B * N) || m.alloc(dspr, 4 * (size_t)B) ||
+        m.alloc(dlen, B) || m.alloc(dnext, B) || m.alloc(dterm, B) || m.alloc(dpart, nwg)) {
+        fprintf(stderr, "argweaver_amd: awamd_arg_prior: device allocation failed\n");
+        return 1;
+    }
+    LNL_OK(hipMemcpy(dm, &model, sizeof(Model), hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dp, ft.parent.data(), sizeof(int) * B * N, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dc0, ft.child0.data(), sizeof(int) * B * N, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dc1, ft.child1.data(), sizeof(int) * B * N, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dag, ft.age.data(), sizeof(int) * B * N, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dspr, ft.spr.data(), sizeof(int) * 4 * B, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dlen, ft.len.data(), sizeof(int) * B, hipMemcpyHostToDevice));
+    LNL_OK(hipMemcpy(dnext, has_next.data(), sizeof(int) * B, hipMemcpyHostToDevice));
+    PriorArgs a{B, N, dm, dp, dc0, dc1, dag, dspr, dlen, dnext, dterm, dpart};
+    hipLaunchKernelGGL(k_prior_block, dim3(nwg), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_sum_terms, dim3(nwg), dim3(256), 0, 0, (const double *)dterm, B, dpart);
+    LNL_OK(hipGetLastError());
+    std::vector<double> part(nwg);
+    LNL_OK(hipMemcpy(part.data(), dpart, sizeof(double) * nwg, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (double x : part) s += x;
+    *out = s;
+    return 0;
+}
+
 }  // namespace
 }  // namespace aw
 
@@ -301,5 +419,27 @@ extern "C" double awamd_arg_likelihood(LocalTrees *trees, const double *times, i
     if (tr.nnodes < 3) return log(.25) * seqlen;
     double v = NAN;
     if (arg_loglik_device(tr, times, ntimes, mu, seqs, nseqs, seqlen, &v)) return NAN;
+    return v;
+}
+
+extern "C" double awamd_arg_prior(LocalTrees *trees, const double *times, int ntimes,
+                                  const double *popsizes, double rho)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fprintf(stderr, "argweaver_amd: awamd_arg_prior needs a HIP device\n");
+        return NAN;
+    }
+    if (ntimes < 3 || ntimes > MAXT) {
+        fprintf(stderr, "argweaver_amd: awamd_arg_prior: ntimes %d outside 3..%d\n", ntimes, MAXT);
+        return NAN;
+    }
+    std::vector<double> pops(ntimes, 1.0);
+    if (popsizes)
+        for (int i = 0; i < ntimes; i++) pops[i] = popsizes[i];
+    Model m;
+    model_init(m, ntimes, times, pops.data(), rho, 0.0);
+    double v = NAN;
+    if (arg_logprior_device(*reinterpret_cast<ArgTrees *>(trees), m, &v)) return NAN;
     return v;
 }

@@ -148,6 +148,12 @@ double arghmm_likelihood(LocalTrees *trees, double *times, int ntimes, double mu
  * NAN without a HIP device or for an ARG over the kernels' limits. */
 double awamd_arg_likelihood(LocalTrees *trees, const double *times, int ntimes, double mu,
                             char **seqs, int nseqs, int seqlen);
+/* log P(ARG) computed on the device (no counterpart in the reference; the
+ * terms of arghmm_prior_prob, calc_arg_prior, total_prob.cpp:262-299, one
+ * per local tree, summed in another order: within 1e-11 relative).  NAN
+ * without a HIP device. */
+double awamd_arg_prior(LocalTrees *trees, const double *times, int ntimes,
+                       const double *popsizes, double rho);
 /* src/argweaver/total_prob.cpp:329-342: aborts, as the reference's does */
 double arghmm_likelihood_parsimony(LocalTrees *trees, double *times, int ntimes, double mu,
                                    char **seqs, int nseqs, int seqlen);
