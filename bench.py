@@ -377,11 +377,18 @@ def mcmc_rethread_fp32(A, seed, log, k=100, L=500000, nsteps=3):
     t2 = time.perf_counter()
     ld = A.arg_likelihood_device(tr, seqs, times, mu)
     t3 = time.perf_counter()
+    t4 = time.perf_counter()
+    ph = A.arg_prior(tr, times, pops, rho)
+    t5 = time.perf_counter()
+    pd = A.arg_prior_device(tr, times, pops, rho)
+    t6 = time.perf_counter()
     lik = dict(host_s=t1 - t0, device_s=t3 - t2, device_first_call_s=t2 - t1,
-               rel_diff=abs(ld - lh) / abs(lh))
+               rel_diff=abs(ld - lh) / abs(lh), prior_host_s=t5 - t4, prior_device_s=t6 - t5,
+               prior_rel_diff=abs(pd - ph) / abs(ph))
     log("log P(seqs | ARG) of the config-5 ARG: host %.3f s, device %.3f s (first call %.3f s), "
-        "relative difference %.1e" % (lik["host_s"], lik["device_s"], lik["device_first_call_s"],
-                                       lik["rel_diff"]))
+        "relative difference %.1e; log P(ARG): host %.3f s, device %.3f s, relative difference "
+        "%.1e" % (lik["host_s"], lik["device_s"], lik["device_first_call_s"], lik["rel_diff"],
+                  lik["prior_host_s"], lik["prior_device_s"], lik["prior_rel_diff"]))
     return dict(steps=steps, seconds_per_step=float(np.mean([s["seconds"] for s in steps[1:]])),
                 fp32_error=err, arg_likelihood=lik)
 
@@ -395,8 +402,9 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
     recombinations and the insertion per chain (host).  The leaves cycle
     k-1, k-2, ..., so steps = k is one sweep re-threading every leaf of
     every chain.  Reports the time per step, the aggregate column-states/s,
-    each chain's recombination count (local trees - 1) and log P(seqs | ARG)
-    (computed on the device, outside the step's time) after every step, and
+    each chain's recombination count (local trees - 1), log P(seqs | ARG) and
+    log P(ARG) (computed on the device, outside the step's time) after every
+    step, and
     its log P(seqs | ARG) and log P(ARG) on the host before and after."""
     times = np.array(A.get_time_points(20, 200e3), np.float64)
     pops, rho, mu = mcmc_model(seed)
@@ -406,7 +414,7 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
     stats0 = [(A.arg_likelihood(tr, sq, times, mu), A.arg_prior(tr, times, pops, rho))
               for tr, sq in chains]
     b = A.Batch()
-    secs, Ws, trace, host_ms, gpu_ms, lik_trace = [], [], [], [], [], []
+    secs, Ws, trace, host_ms, gpu_ms, lik_trace, prior_trace = [], [], [], [], [], [], []
     # the chains' host work runs on one thread per chain (the library's
     # calls release the interpreter lock); the recombination draws of all
     # chains come from the one process-global rand() stream, so they run in
@@ -438,6 +446,7 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
         # log P(seqs | ARG) of every chain after the step, on the device
         # (outside the step's time)
         lik_trace.append([A.arg_likelihood_device(tr, sq, times, mu) for tr, sq in chains])
+        prior_trace.append([A.arg_prior_device(tr, times, pops, rho) for tr, _ in chains])
         if j % 20 == 19 or j == steps - 1:
             log("mcmc batched chains: step %d/%d, %.2f s per step (host %.0f ms, batch run %.0f "
                 "ms), recombinations %s" % (j + 1, steps, float(np.mean(secs[-20:])),
@@ -455,6 +464,7 @@ def mcmc_batched_chains(A, seed, log, nchains=4, k=100, L=500000, steps=100):
                batch_run_ms_per_step=float(np.mean(gpu_ms[1:] or gpu_ms)),
                column_states_per_s=float(np.mean(Ws[1:] or Ws)) / sps,
                recombinations_trace=trace, log_likelihood_trace=lik_trace,
+               log_prior_trace=prior_trace,
                log_likelihood_device_vs_host_end=[
                    abs(lik_trace[-1][c] - stats1[c][0]) / abs(stats1[c][0]) for c in range(nchains)],
                log_likelihood_start=[x[0] for x in stats0], log_likelihood_end=[x[0] for x in stats1],
