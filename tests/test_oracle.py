@@ -163,7 +163,7 @@ def test_oracle_forward_matches_compiled_reference(aw, k, L, seed):
 
 @needs_ref
 @pytest.mark.parametrize("seed,k,L", [(1, 8, 3000), (2, 8, 3000), (3, 8, 3000), (4, 8, 3000),
-                                      (5, 24, 60000)])
+                                      (5, 24, 60000), (6, 40, 40000), (7, 12, 150000)])
 def test_sample_thread_chain_matches_compiled_reference(aw, seed, k, L):
     """oracle traceback + recombination sampling + this library's add_arg_thread
     reproduce the reference's arghmm_sample_thread node for node.  The last
